@@ -1,0 +1,142 @@
+/*
+ * wavedm.h -- C ABI of libwavedm_hip.so, the MI355X (gfx950) backend for the
+ * split-step Fourier (SSFM) kick-drift step and FFT Poisson gravity of
+ * WaveDM.jl.
+ *
+ * Plain C types only.  All arrays are column-major / x-fastest (Julia layout:
+ * psi[i + nx*(j + ny*k)]), complex values interleaved (re, im), exactly what
+ * Julia's pointer(A) hands over for Array{ComplexF64,3} / Array{ComplexF32,3}.
+ *
+ * Every function returns WDM_OK (0) or a negative WDM_ERR_* code; the message
+ * is available from wdm_last_error(plan) (or wdm_last_error(NULL) for errors
+ * raised before a plan exists).  No exceptions cross the ABI.
+ *
+ * Threading: a plan owns one HIP stream; a plan is not thread-safe, distinct
+ * plans are independent.  Calls return after the GPU work they issued has
+ * completed.
+ *
+ * Reference interfaces replaced (paths relative to the WaveDM.jl tree):
+ *   wdm_plan_create      setup_grid / setup_coordinates   src/physics/poisson.jl:8-28
+ *                        + the driver's buffer set-up      src/simulation_main.jl:212-214,414-417
+ *   wdm_set_psi          DeviceArray(psi) upload           src/simulation_main.jl:417
+ *   wdm_get_psi          collect(device_psi)               src/simulation_main.jl:551
+ *   wdm_set_border       setup_absorption_boundary         src/solver/fft_operators.jl:22-27
+ *   wdm_set_static_potential / wdm_set_potential_model
+ *                        the V(x,y,z,psi) callback         src/solver/KDK.jl:64
+ *   wdm_poisson          parallel_poisson / compute_gravitational_potential
+ *                                                          src/parallel_poisson.jl:36-68,
+ *                                                          src/solver/gravity.jl:7-35
+ *   wdm_get_phi          Phi_all returned by the kick      src/solver/KDK.jl:54-55
+ *   wdm_kick             apply_kick_step!                  src/solver/KDK.jl:37-77
+ *   wdm_drift            apply_drift_step! (+ setup_fft_operators, fft_operators.jl:8-15)
+ *                                                          src/solver/KDK.jl:88-101
+ *   wdm_step             the loop body, n iterations       src/simulation_main.jl:536-551
+ *   wdm_cfl              compute_timestep limits           src/physics/poisson.jl:35-50
+ *   wdm_reduce           mass / energy diagnostics         src/simulation_main.jl:552-558,
+ *                                                          src/visualization/runtime.jl:176-179
+ */
+#ifndef WAVEDM_H
+#define WAVEDM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WDM_ABI_VERSION 1
+
+/* status codes */
+#define WDM_OK 0
+#define WDM_ERR_ARG (-1)
+#define WDM_ERR_OOM (-2)
+#define WDM_ERR_HIP (-3)
+#define WDM_ERR_NONFINITE (-4)
+#define WDM_ERR_UNSUPPORTED (-5)
+#define WDM_ERR_STATE (-6)
+
+/* plan precision */
+#define WDM_F64 0 /* complex128 FFTs and phases (the plain form) */
+#define WDM_F32 1 /* complex64 FFTs; FP64 phase reduction, FP64 reductions */
+
+/* host/device array element types */
+#define WDM_C128 0
+#define WDM_C64 1
+
+/* plan flags */
+#define WDM_GRID_CONSISTENT 1u /* Delta = 2*Xmax/N instead of the reference 2*Xmax/(N-1) */
+
+/* split modes for wdm_step */
+#define WDM_SPLIT_REFERENCE 0 /* K(dt_kick) D(dt) per iteration (src/simulation_main.jl:536-547) */
+#define WDM_SPLIT_STRANG 1    /* K(dt/2) D(dt) K(dt/2) with Phi recomputed after the drift */
+
+/* potential models (wdm_set_potential_model kind) */
+#define WDM_V_HERNQUIST 2  /* params: M, a, cx, cy, cz          V = -G M / (|x-c| + a) */
+#define WDM_V_QUADRUPOLE 4 /* params: Txx,Tyy,Tzz,Txy,Txz,Tyz, nx,ny,nz, period, cx,cy,cz:
+                              V = 1/2 (x-c)^T R(t) T R(t)^T (x-c), R(t) rotation about n
+                              by 2 pi t / period */
+#define WDM_V_KAPPA 8      /* params: kappa                      V = kappa |psi|^2 */
+
+/* wdm_reduce output slots */
+#define WDM_RED_MASS 0    /* sum |psi|^2 dV                                    */
+#define WDM_RED_KINETIC 1 /* (hbar/m)^2/2 sum_k k^2 |psi_k|^2 dV / N           */
+#define WDM_RED_WSELF 2   /* 1/2 sum |psi|^2 Phi dV                            */
+#define WDM_RED_VEXT 3    /* sum |psi|^2 V dV (V at the last kick time)        */
+#define WDM_RED_PSIMAX 4  /* max |psi|                                         */
+#define WDM_RED_PHIMAX 5  /* max |Phi|                                         */
+#define WDM_RED_COUNT 6
+
+typedef struct wdm_plan wdm_plan;
+
+int wdm_version(void);
+const char* wdm_last_error(const wdm_plan* plan);
+
+/* Grid nx*ny*nz on [-xmax, xmax] x ...; periodic box length 2*xmax per axis. */
+int wdm_plan_create(wdm_plan** plan, int nx, int ny, int nz, double xmax, double ymax, double zmax,
+                    int precision, unsigned flags);
+int wdm_plan_destroy(wdm_plan* plan);
+
+/* hbar/m (drift and kick phases) and 4 pi G (Poisson source); the reference's
+   code units are hbar_over_m = 1, four_pi_G = 4 pi. */
+int wdm_set_coeffs(wdm_plan* plan, double hbar_over_m, double four_pi_G);
+int wdm_set_self_gravity(wdm_plan* plan, int on);
+
+int wdm_set_psi(wdm_plan* plan, const void* host_psi, int dtype);
+int wdm_get_psi(wdm_plan* plan, void* host_psi, int dtype);
+/* device-pointer variants (memory from hipMalloc / a torch tensor on this GPU) */
+int wdm_set_psi_device(wdm_plan* plan, const void* dev_psi, int dtype);
+int wdm_get_psi_device(wdm_plan* plan, void* dev_psi, int dtype);
+
+/* rank 1: length-nx FP64 vector applied along x after each drift (the
+   reference's broadcast, src/simulation_main.jl:415).  NULL clears. */
+int wdm_set_border(wdm_plan* plan, const double* border, int rank);
+/* full-grid FP64 static potential added to Phi in every kick; NULL clears. */
+int wdm_set_static_potential(wdm_plan* plan, const double* V);
+int wdm_set_potential_model(wdm_plan* plan, int kind, const double* params, int nparams);
+int wdm_clear_potential_models(wdm_plan* plan);
+
+/* phi_out = scale * lap^-1 (rho - <rho>) on the plan's periodic lattice
+   (k = 2 pi m / (2 xmax), k = 0 -> 0).  rho == NULL: rho = |psi|^2 of the
+   plan's psi.  phi_out == NULL: keep the result on the device for the next
+   kick (scale must then be 4 pi G, pass 0 to use the plan's coefficient). */
+int wdm_poisson(wdm_plan* plan, const double* rho, double* phi_out, double scale);
+/* real-space Phi (4 pi G lap^-1 (|psi|^2 - mean)) of the current psi, FP64 */
+int wdm_get_phi(wdm_plan* plan, double* host_phi);
+
+/* apply_kick_step!: psi <- FFT[exp(-i (m/hbar) dt_kick (Phi + V(t))) psi];
+   afterwards the plan holds the spectrum (get_psi returns it). */
+int wdm_kick(wdm_plan* plan, double dt_kick, double t);
+/* apply_drift_step!: psi <- border .* IFFT[exp(-i (hbar/m) k^2 dt/2) psi_k] */
+int wdm_drift(wdm_plan* plan, double dt);
+/* nsteps iterations of the fused step; kick i happens at t0 + i*dt. */
+int wdm_step(wdm_plan* plan, int nsteps, double dt, double dt_kick, double t0, int split_mode);
+
+/* out[0..min(nout,WDM_RED_COUNT)) = WDM_RED_* diagnostics of the current psi */
+int wdm_reduce(wdm_plan* plan, double* out, int nout);
+/* out3 = {2 dx^2/(pi hbar/m), 2 pi/((m/hbar) max|Phi|), 2 pi/((m/hbar)|kappa| max|psi|)} */
+int wdm_cfl(wdm_plan* plan, double kappa, double* out3);
+
+int wdm_synchronize(wdm_plan* plan);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WAVEDM_H */

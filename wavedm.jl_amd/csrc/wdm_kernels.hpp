@@ -1,0 +1,610 @@
+// Pass kernels of the fused SSFM step.  Arrays are x-fastest (Julia column
+// order): psi[x + Nx*(y + Ny*z)]; the Poisson half spectrum is
+// rho_hat[kx + Hp*(y + Ny*z)] with kx in [0, Nx/2] and row pitch Hp.
+//
+// Steady-state step (reference split, src/simulation_main.jl:536-551):
+//   k_x_kick      x: Phi = C2R_x(Phi_hat) ; psi *= exp(-i (m/hbar) dt_kick (Phi+V)) ; FFT_x
+//   k_col FWD     y: FFT_y
+//   k_col FMI     z: FFT_z * drift(k) / N * IFFT_z
+//   k_col INV     y: IFFT_y
+//   k_x_inv_dens  x: IFFT_x ; * border ; store psi ; rho=|psi|^2 ; R2C_x -> rho_hat
+//   k_col FWD     y, k_col FMI z (* -4 pi G / k^2 / N), k_col INV y  on rho_hat
+// so the next step's kick finds Phi_hat ready.
+#pragma once
+
+#include "wdm_lines.hpp"
+
+namespace wdm {
+
+enum ColMode { COL_FWD = 0, COL_INV = 1, COL_FMI = 2, COL_MI = 3 };
+// KF_PNONE: no factor, Plain arithmetic (the y passes of the Poisson chain)
+enum KFactor { KF_NONE = 0, KF_DRIFT = 1, KF_POISSON = 2, KF_PNONE = 3 };
+
+// k-space factor tables for the line, column and "other" axes of a pass.
+//   KF_DRIFT:   complex FP64 exp(-i (hbar/m) k_a^2 dt / 2) per axis (interleaved),
+//               multiplied together and by `scale` (= 1/N).
+//   KF_POISSON: real FP64 k_a^2 per axis; factor = scale / (kc^2+ko^2+kl^2),
+//               0 at k = 0 (scale = -4 pi G / N).
+struct KTabs {
+    const double* col;
+    const double* oth;
+    const double* line;
+    double scale;
+};
+
+template <int KF, class P, class T>
+WDM_FI cx<T> apply_kf(cx<T> v, const KTabs& kt, int c, int o, int k) {
+    if constexpr (KF == KF_DRIFT) {
+        const double ar = kt.col[2 * c], ai = kt.col[2 * c + 1];
+        const double br = kt.oth[2 * o], bi = kt.oth[2 * o + 1];
+        const double cr = kt.line[2 * k], ci = kt.line[2 * k + 1];
+        const double abr = ar * br - ai * bi, abi = ar * bi + ai * br;
+        const double re = (abr * cr - abi * ci) * kt.scale;
+        const double im = (abr * ci + abi * cr) * kt.scale;
+        return mul_dyn<P>(v, re, im);
+    } else if constexpr (KF == KF_POISSON) {
+        const double s = kt.col[c] + kt.oth[o] + kt.line[k];
+        const double g = s > 0.0 ? kt.scale / s : 0.0;
+        return v * T(g);
+    } else {
+        return v;
+    }
+}
+
+// ------------------------------------------------------------------ potentials
+// External potential models evaluated in-kernel (replacing the host V(x,y,z,psi)
+// callback of src/solver/KDK.jl:64).  All parameters FP64.
+enum VFlags { V_STATIC = 1, V_HERNQUIST = 2, V_QUADRUPOLE = 4, V_KAPPA = 8 };
+struct VModel {
+    int flags;
+    const double* vstatic;   // full grid, x-fastest (V_STATIC)
+    double hq_GM, hq_a, hq_c[3];  // V = -GM / (|x - c| + a)
+    double qT[6];            // Txx Tyy Tzz Txy Txz Tyz at the kick time; V = 1/2 (x-c)^T T (x-c)
+    double q_c[3];
+    double kappa;            // V = kappa |psi|^2 (GNLSE term)
+};
+
+template <class T>
+WDM_FI double eval_V(const VModel& vm, double x, double y, double z, unsigned idx, cx<T> psi) {
+    double V = 0.0;
+    if (vm.flags & V_STATIC) V += vm.vstatic[idx];
+    if (vm.flags & V_HERNQUIST) {
+        const double dx = x - vm.hq_c[0], dy = y - vm.hq_c[1], dz = z - vm.hq_c[2];
+        V -= vm.hq_GM / (__builtin_sqrt(dx * dx + dy * dy + dz * dz) + vm.hq_a);
+    }
+    if (vm.flags & V_QUADRUPOLE) {
+        const double dx = x - vm.q_c[0], dy = y - vm.q_c[1], dz = z - vm.q_c[2];
+        V += 0.5 * (vm.qT[0] * dx * dx + vm.qT[1] * dy * dy + vm.qT[2] * dz * dz) +
+             vm.qT[3] * dx * dy + vm.qT[4] * dx * dz + vm.qT[5] * dy * dz;
+    }
+    if (vm.flags & V_KAPPA) V += vm.kappa * double(norm2(psi));
+    return V;
+}
+
+// ------------------------------------------------------------------ phases
+// Reduce theta to r in [-pi/4, pi/4] and quadrant q (theta = q*pi/2 + r), FP64.
+WDM_FI double reduce_quadrant(double theta, int& q) {
+    constexpr double kTwoOverPi = 0.63661977236758134307553505349006;
+    constexpr double kPio2Hi = 1.5707963267948965579989817342721;
+    constexpr double kPio2Lo = 6.1232339957367660359076653e-17;
+    const double n = __builtin_rint(theta * kTwoOverPi);
+    double r = fmaT(-n, kPio2Hi, theta);
+    r = fmaT(-n, kPio2Lo, r);
+    q = int(long(n) & 3);
+    return r;
+}
+
+// sin and cos on |r| <= pi/4 by Taylor polynomials in r^2 (enough terms for the
+// working precision of T).
+WDM_FI void sincos_small(double r, double& s, double& c) {
+    const double r2 = r * r;
+    // sin: r * sum_{k=0..8} (-1)^k r^(2k)/(2k+1)!
+    double ps = -1.0 / 355687428096000.0;      // -1/17!
+    ps = fmaT(ps, r2, 1.0 / 1307674368000.0);  // 1/15!
+    ps = fmaT(ps, r2, -1.0 / 6227020800.0);    // -1/13!
+    ps = fmaT(ps, r2, 1.0 / 39916800.0);       // 1/11!
+    ps = fmaT(ps, r2, -1.0 / 362880.0);        // -1/9!
+    ps = fmaT(ps, r2, 1.0 / 5040.0);           // 1/7!
+    ps = fmaT(ps, r2, -1.0 / 120.0);           // -1/5!
+    ps = fmaT(ps, r2, 1.0 / 6.0);              // 1/3!
+    s = fmaT(-ps * r2, r, r);
+    // cos: sum_{k=0..9} (-1)^k r^(2k)/(2k)!
+    double pc = 1.0 / 6402373705728000.0;      // 1/18!
+    pc = fmaT(pc, r2, -1.0 / 20922789888000.0);  // -1/16!
+    pc = fmaT(pc, r2, 1.0 / 87178291200.0);    // 1/14!
+    pc = fmaT(pc, r2, -1.0 / 479001600.0);     // -1/12!
+    pc = fmaT(pc, r2, 1.0 / 3628800.0);        // 1/10!
+    pc = fmaT(pc, r2, -1.0 / 40320.0);         // -1/8!
+    pc = fmaT(pc, r2, 1.0 / 720.0);            // 1/6!
+    pc = fmaT(pc, r2, -1.0 / 24.0);            // -1/4!
+    pc = fmaT(pc, r2, 0.5);                    // 1/2!
+    c = fmaT(-pc, r2, 1.0);
+}
+WDM_FI void sincos_small(float r, float& s, float& c) {
+    const float r2 = r * r;
+    float ps = 2.7557319e-6f;            // 1/9!
+    ps = fmaT(ps, r2, -1.98412698e-4f);  // -1/7!
+    ps = fmaT(ps, r2, 8.33333333e-3f);   // 1/5!
+    ps = fmaT(ps, r2, -1.66666667e-1f);  // -1/3!
+    s = fmaT(ps * r2, r, r);
+    float pc = -2.75573192e-7f;          // -1/10!
+    pc = fmaT(pc, r2, 2.48015873e-5f);   // 1/8!
+    pc = fmaT(pc, r2, -1.38888889e-3f);  // -1/6!
+    pc = fmaT(pc, r2, 4.16666667e-2f);   // 1/4!
+    pc = fmaT(pc, r2, -0.5f);            // -1/2!
+    c = fmaT(pc, r2, 1.0f);
+}
+
+// multiply by (-i)^q
+template <class T>
+WDM_FI cx<T> rot_quadrant(cx<T> a, int q) {
+    const cx<T> b = (q & 1) ? mk(a.y, -a.x) : a;
+    return (q & 2) ? -b : b;
+}
+
+// a * exp(-i theta).  FP64: direct product.  FP32: theta reduced in FP64, sin
+// and cos in FP32, applied as three shears (x -= t y; y += s x; x -= t y with
+// t = tan(phi/2), s = sin(phi)); the shear product has determinant exactly 1
+// whatever t and s round to, so the kick adds no systematic bias to |psi|^2.
+WDM_FI cx<double> kick_rot(cx<double> a, double theta) {
+    int q;
+    const double r = reduce_quadrant(theta, q);
+    double s, c;
+    sincos_small(r, s, c);
+    const cx<double> b = mk(fmaT(a.x, c, a.y * s), fmaT(a.y, c, -(a.x * s)));
+    return rot_quadrant(b, q);
+}
+WDM_FI cx<float> kick_rot(cx<float> a, double theta) {
+    int q;
+    const float r = float(reduce_quadrant(theta, q));
+    float s, c;
+    sincos_small(r, s, c);
+    // rotation by phi = -r
+    const float sp = -s;
+    const float tp = -s / (1.0f + c);
+    float x = a.x, y = a.y;
+    x = fmaT(-tp, y, x);
+    y = fmaT(sp, x, y);
+    x = fmaT(-tp, y, x);
+    return rot_quadrant(mk(x, y), q);
+}
+
+// multiply by a real FP64 factor in data precision (FP32 compensated)
+template <class P>
+WDM_FI cx<double> scale_dyn(cx<double> a, double f) { return a * f; }
+template <class P>
+WDM_FI cx<float> scale_dyn(cx<float> a, double f) {
+    if constexpr (std::is_same<P, Plain>::value) {
+        return a * float(f);
+    } else {
+        const float h = float(f), l = float(f - double(h));
+        return mk(fmaT(a.x, h, a.x * l), fmaT(a.y, h, a.y * l));
+    }
+}
+
+// ------------------------------------------------------------------ wave helpers
+struct LaneMap {
+    int lane, wv, j, lw, w;
+};
+template <class F, int NW>
+WDM_FI LaneMap lane_map() {
+    LaneMap m;
+    m.lane = int(threadIdx.x) & (kWave - 1);
+    m.wv = int(threadIdx.x) / kWave;
+    m.j = m.lane % F::N2;
+    m.lw = m.lane / F::N2;
+    m.w = int(blockIdx.x) * NW + m.wv;
+    return m;
+}
+
+// Deterministic wave sum/max of per-lane FP64 values through the wave's LDS
+// buffer; lane 0 returns the result.
+template <class T>
+WDM_FI void wave_sum_max(cx<T>* wbuf, int lane, double vsum, double vmax, double& osum, double& omax) {
+    double* d = reinterpret_cast<double*>(wbuf);
+    d[lane] = vsum;
+    d[kWave + lane] = vmax;
+    WDM_WAVE_SYNC();
+    osum = 0.0;
+    omax = 0.0;
+    if (lane == 0) {
+        for (int l = 0; l < kWave; ++l) {
+            osum += d[l];
+            omax = d[kWave + l] > omax ? d[kWave + l] : omax;
+        }
+    }
+    WDM_WAVE_SYNC();
+}
+
+// ------------------------------------------------------------------ column pass (y or z)
+// Lines along an axis with element stride `sline`; columns are the contiguous
+// x index c in [0, ncols); `nother` positions of the remaining axis at stride
+// `sother`.  Each wave owns LPW adjacent columns at one `other` position.
+template <class T, class P, class F, int MODE, int KF, int NW>
+WDM_KERNEL(NW * 64)
+void k_col(cx<T>* __restrict__ data, int ncols, int nother, unsigned sline, unsigned sother,
+           const typename TwEntry<T>::type* __restrict__ tw, KTabs kt) {
+    constexpr int N1 = F::N1, N2 = F::N2, M = F::M;
+    WDM_SHARED cx<T> xbuf[NW][F::XBUF];
+    const LaneMap m = lane_map<F, NW>();
+    const int ntiles = (ncols + F::LPW - 1) / F::LPW;
+    const int o = m.w / ntiles, tile = m.w % ntiles;
+    const int c = tile * F::LPW + m.lw;
+    const bool ok = (o < nother) && (c < ncols);
+    const int oc = ok ? o : 0, cc = ok ? c : 0;
+    cx<T>* base = data + size_t(oc) * sother + cc;
+    cx<T>* xb = &xbuf[m.wv][m.lw * F::XS];
+    const cx<T> zero = mk(T(0), T(0));
+    cx<T> v[N1];
+    cx<T> X[M][N2];
+    if constexpr (MODE == COL_MI) {
+        WDM_UNROLL
+        for (int n1 = 0; n1 < N1; ++n1) {
+            const int k = N2 * n1 + m.j;
+            v[n1] = ok ? apply_kf<KF, P, T>(base[unsigned(k) * sline], kt, cc, oc, k) : zero;
+        }
+    } else {
+        WDM_UNROLL
+        for (int n1 = 0; n1 < N1; ++n1) v[n1] = ok ? base[unsigned(N2 * n1 + m.j) * sline] : zero;
+    }
+    constexpr int DIR = (MODE == COL_FWD || MODE == COL_FMI) ? -1 : +1;
+    fwd2<F, DIR, P>(v, X, m.j, xb, tw);
+    if constexpr (MODE == COL_FMI) {
+        WDM_UNROLL
+        for (int i = 0; i < M; ++i)
+            WDM_UNROLL
+            for (int k2 = 0; k2 < N2; ++k2)
+                X[i][k2] = apply_kf<KF, P, T>(X[i][k2], kt, cc, oc, (i * N2 + m.j) + N1 * k2);
+        inv2<F, +1, P>(X, v, m.j, xb, tw);
+        if (ok) {
+            WDM_UNROLL
+            for (int kq = 0; kq < N1; ++kq) base[unsigned(N2 * kq + m.j) * sline] = v[kq];
+        }
+    } else {
+        if (ok) {
+            WDM_UNROLL
+            for (int i = 0; i < M; ++i)
+                WDM_UNROLL
+                for (int k2 = 0; k2 < N2; ++k2) base[unsigned((i * N2 + m.j) + N1 * k2) * sline] = X[i][k2];
+        }
+    }
+}
+
+// ------------------------------------------------------------------ x-passes
+struct KickArgs {
+    double mh_dt;        // (m/hbar) * dt_kick
+    double t;            // kick time (for time-dependent models, folded into VModel by the host)
+    int use_phi;         // self-gravity on
+    int Ny;
+    const double* xs;    // coordinates per axis (FP64)
+    const double* ys;
+    const double* zs;
+    VModel vm;
+};
+
+// Kick (+ optional forward x-FFT).  Reads psi rows in real space and Phi_hat
+// (half spectrum after the y/z Poisson passes); writes psi (x-transformed when
+// FWD) in place.
+template <class T, class P, class F, bool FWD, int NW>
+WDM_KERNEL(NW * 64)
+void k_x_kick(cx<T>* __restrict__ psi, const cx<T>* __restrict__ phihat, int nlines, int hpitch,
+              const typename TwEntry<T>::type* __restrict__ tw, KickArgs ka) {
+    constexpr int N1 = F::N1, N2 = F::N2, M = F::M, N = F::N;
+    WDM_SHARED cx<T> xbuf[NW][F::XBUF];
+    const LaneMap m = lane_map<F, NW>();
+    const int l = m.w * F::LPW + m.lw;
+    const bool ok = l < nlines;
+    const int lc = ok ? l : 0;
+    const int y = lc % ka.Ny, z = lc / ka.Ny;
+    cx<T>* row = psi + size_t(lc) * N;
+    cx<T>* xb = &xbuf[m.wv][m.lw * F::XS];
+    const cx<T> zero = mk(T(0), T(0));
+    cx<T> X[M][N2];
+    cx<T> v[N1];
+    T phi[N1];
+    if (ka.use_phi) {
+        const cx<T>* h = phihat + size_t(lc) * hpitch;
+        WDM_UNROLL
+        for (int i = 0; i < M; ++i)
+            WDM_UNROLL
+            for (int p = 0; p < N2; ++p) {
+                const int k = i * N2 + m.j + N1 * p;
+                cx<T> zz;
+                if (2 * k <= N) {
+                    zz = h[k];
+                } else {
+                    const cx<T> hh = h[N - k];
+                    zz = mk(hh.x, -hh.y);
+                }
+                X[i][p] = ok ? zz : zero;
+            }
+        inv2<F, +1, Plain>(X, v, m.j, xb, tw);
+        WDM_UNROLL
+        for (int kq = 0; kq < N1; ++kq) phi[kq] = v[kq].x;
+    } else {
+        WDM_UNROLL
+        for (int kq = 0; kq < N1; ++kq) phi[kq] = T(0);
+    }
+    const double yc = ka.ys[y], zc = ka.zs[z];
+    WDM_UNROLL
+    for (int n1 = 0; n1 < N1; ++n1) {
+        const int x = N2 * n1 + m.j;
+        const cx<T> a = ok ? row[x] : zero;
+        const double V = ka.vm.flags ? eval_V<T>(ka.vm, ka.xs[x], yc, zc, unsigned(lc) * N + x, a) : 0.0;
+        v[n1] = kick_rot(a, ka.mh_dt * (double(phi[n1]) + V));
+    }
+    if constexpr (FWD) {
+        fwd2<F, -1, P>(v, X, m.j, xb, tw);
+        if (ok) {
+            WDM_UNROLL
+            for (int i = 0; i < M; ++i)
+                WDM_UNROLL
+                for (int k2 = 0; k2 < N2; ++k2) row[(i * N2 + m.j) + N1 * k2] = X[i][k2];
+        }
+    } else {
+        if (ok) {
+            WDM_UNROLL
+            for (int n1 = 0; n1 < N1; ++n1) row[N2 * n1 + m.j] = v[n1];
+        }
+    }
+}
+
+// rho = |psi|^2 of real-space values v[] (positions x = N2*kq + j), forward R2C
+// along x into the half-spectrum row h, plus FP64 sum/max partials per wave.
+template <class T, class F, int NW>
+WDM_FI void density_r2c(const cx<T>* v, bool ok, const LaneMap& m, cx<T>* xb, cx<T>* wbuf,
+                        const typename TwEntry<T>::type* __restrict__ tw, cx<T>* h,
+                        double* __restrict__ partials) {
+    constexpr int N1 = F::N1, N2 = F::N2, M = F::M, N = F::N;
+    cx<T> r[N1];
+    double msum = 0.0, mmax = 0.0;
+    WDM_UNROLL
+    for (int kq = 0; kq < N1; ++kq) {
+        const T rr = norm2(v[kq]);
+        r[kq] = mk(rr, T(0));
+        msum += double(rr);
+        mmax = double(rr) > mmax ? double(rr) : mmax;
+    }
+    if (!ok) { msum = 0.0; mmax = 0.0; }
+    cx<T> X[M][N2];
+    fwd2<F, -1, Plain>(r, X, m.j, xb, tw);
+    if (ok) {
+        WDM_UNROLL
+        for (int i = 0; i < M; ++i)
+            WDM_UNROLL
+            for (int k2 = 0; k2 < N2; ++k2) {
+                const int k = (i * N2 + m.j) + N1 * k2;
+                if (2 * k <= N) h[k] = X[i][k2];
+            }
+    }
+    if (partials) {
+        double s, mx;
+        wave_sum_max(wbuf, m.lane, msum, mmax, s, mx);
+        if (m.lane == 0) {
+            partials[2 * m.w] = s;
+            partials[2 * m.w + 1] = mx;
+        }
+    }
+}
+
+// Inverse x-FFT (+ absorber along x) of psi rows, store psi in real space,
+// then rho = |psi|^2 -> R2C_x -> rho_hat.  `border` is a length-Nx FP64
+// vector or null (src/solver/KDK.jl:95-96; broadcast along x, Q4).
+template <class T, class P, class F, int NW>
+WDM_KERNEL(NW * 64)
+void k_x_inv_dens(cx<T>* __restrict__ psi, cx<T>* __restrict__ rhohat, int nlines, int hpitch,
+                  const typename TwEntry<T>::type* __restrict__ tw, const double* __restrict__ border,
+                  double* __restrict__ partials) {
+    constexpr int N1 = F::N1, N2 = F::N2, M = F::M, N = F::N;
+    WDM_SHARED cx<T> xbuf[NW][F::XBUF];
+    const LaneMap m = lane_map<F, NW>();
+    const int l = m.w * F::LPW + m.lw;
+    const bool ok = l < nlines;
+    const int lc = ok ? l : 0;
+    cx<T>* row = psi + size_t(lc) * N;
+    cx<T>* xb = &xbuf[m.wv][m.lw * F::XS];
+    const cx<T> zero = mk(T(0), T(0));
+    cx<T> X[M][N2];
+    cx<T> v[N1];
+    WDM_UNROLL
+    for (int i = 0; i < M; ++i)
+        WDM_UNROLL
+        for (int p = 0; p < N2; ++p) X[i][p] = ok ? row[(i * N2 + m.j) + N1 * p] : zero;
+    inv2<F, +1, P>(X, v, m.j, xb, tw);
+    if (border) {
+        WDM_UNROLL
+        for (int kq = 0; kq < N1; ++kq) v[kq] = scale_dyn<P>(v[kq], border[N2 * kq + m.j]);
+    }
+    if (ok) {
+        WDM_UNROLL
+        for (int kq = 0; kq < N1; ++kq) row[N2 * kq + m.j] = v[kq];
+    }
+    if (rhohat) {
+        density_r2c<T, F, NW>(v, ok, m, xb, &xbuf[m.wv][0], tw, rhohat + size_t(lc) * hpitch, partials);
+    }
+}
+
+// rho_hat (and sum/max partials) from psi in real space (start of a run, or
+// after the host replaced psi).
+template <class T, class F, int NW>
+WDM_KERNEL(NW * 64)
+void k_x_dens(const cx<T>* __restrict__ psi, cx<T>* __restrict__ rhohat, int nlines, int hpitch,
+              const typename TwEntry<T>::type* __restrict__ tw, double* __restrict__ partials) {
+    constexpr int N1 = F::N1, N2 = F::N2, N = F::N;
+    WDM_SHARED cx<T> xbuf[NW][F::XBUF];
+    const LaneMap m = lane_map<F, NW>();
+    const int l = m.w * F::LPW + m.lw;
+    const bool ok = l < nlines;
+    const int lc = ok ? l : 0;
+    const cx<T>* row = psi + size_t(lc) * N;
+    cx<T>* xb = &xbuf[m.wv][m.lw * F::XS];
+    const cx<T> zero = mk(T(0), T(0));
+    cx<T> v[N1];
+    WDM_UNROLL
+    for (int n1 = 0; n1 < N1; ++n1) v[n1] = ok ? row[N2 * n1 + m.j] : zero;
+    density_r2c<T, F, NW>(v, ok, m, xb, &xbuf[m.wv][0], tw, rhohat + size_t(lc) * hpitch, partials);
+}
+
+// R2C_x of a real FP64 source array (standalone Poisson solve of a host rho).
+template <class T, class F, int NW>
+WDM_KERNEL(NW * 64)
+void k_x_r2c_real(const double* __restrict__ src, cx<T>* __restrict__ rhohat, int nlines, int hpitch,
+                  const typename TwEntry<T>::type* __restrict__ tw) {
+    constexpr int N1 = F::N1, N2 = F::N2, M = F::M, N = F::N;
+    WDM_SHARED cx<T> xbuf[NW][F::XBUF];
+    const LaneMap m = lane_map<F, NW>();
+    const int l = m.w * F::LPW + m.lw;
+    const bool ok = l < nlines;
+    const int lc = ok ? l : 0;
+    const double* row = src + size_t(lc) * N;
+    cx<T>* xb = &xbuf[m.wv][m.lw * F::XS];
+    cx<T> v[N1];
+    WDM_UNROLL
+    for (int n1 = 0; n1 < N1; ++n1) v[n1] = mk(ok ? T(row[N2 * n1 + m.j]) : T(0), T(0));
+    cx<T> X[M][N2];
+    fwd2<F, -1, Plain>(v, X, m.j, xb, tw);
+    if (ok) {
+        cx<T>* h = rhohat + size_t(lc) * hpitch;
+        WDM_UNROLL
+        for (int i = 0; i < M; ++i)
+            WDM_UNROLL
+            for (int k2 = 0; k2 < N2; ++k2) {
+                const int k = (i * N2 + m.j) + N1 * k2;
+                if (2 * k <= N) h[k] = X[i][k2];
+            }
+    }
+}
+
+// C2R_x of a half spectrum into a real FP64 array (Phi on the grid).
+template <class T, class F, int NW>
+WDM_KERNEL(NW * 64)
+void k_x_c2r_real(const cx<T>* __restrict__ hsrc, double* __restrict__ dst, int nlines, int hpitch,
+                  const typename TwEntry<T>::type* __restrict__ tw) {
+    constexpr int N1 = F::N1, N2 = F::N2, M = F::M, N = F::N;
+    WDM_SHARED cx<T> xbuf[NW][F::XBUF];
+    const LaneMap m = lane_map<F, NW>();
+    const int l = m.w * F::LPW + m.lw;
+    const bool ok = l < nlines;
+    const int lc = ok ? l : 0;
+    const cx<T>* h = hsrc + size_t(lc) * hpitch;
+    cx<T>* xb = &xbuf[m.wv][m.lw * F::XS];
+    const cx<T> zero = mk(T(0), T(0));
+    cx<T> X[M][N2];
+    cx<T> v[N1];
+    WDM_UNROLL
+    for (int i = 0; i < M; ++i)
+        WDM_UNROLL
+        for (int p = 0; p < N2; ++p) {
+            const int k = i * N2 + m.j + N1 * p;
+            cx<T> zz;
+            if (2 * k <= N) {
+                zz = h[k];
+            } else {
+                const cx<T> hh = h[N - k];
+                zz = mk(hh.x, -hh.y);
+            }
+            X[i][p] = ok ? zz : zero;
+        }
+    inv2<F, +1, Plain>(X, v, m.j, xb, tw);
+    if (ok) {
+        double* row = dst + size_t(lc) * N;
+        WDM_UNROLL
+        for (int kq = 0; kq < N1; ++kq) row[N2 * kq + m.j] = double(v[kq].x);
+    }
+}
+
+// Plain x-FFT of psi rows in place (DIR -1 forward, +1 inverse, unnormalised).
+template <class T, class P, class F, int DIR, int NW>
+WDM_KERNEL(NW * 64)
+void k_x_fft(cx<T>* __restrict__ psi, int nlines, const typename TwEntry<T>::type* __restrict__ tw) {
+    constexpr int N1 = F::N1, N2 = F::N2, M = F::M, N = F::N;
+    WDM_SHARED cx<T> xbuf[NW][F::XBUF];
+    const LaneMap m = lane_map<F, NW>();
+    const int l = m.w * F::LPW + m.lw;
+    const bool ok = l < nlines;
+    const int lc = ok ? l : 0;
+    cx<T>* row = psi + size_t(lc) * N;
+    cx<T>* xb = &xbuf[m.wv][m.lw * F::XS];
+    const cx<T> zero = mk(T(0), T(0));
+    cx<T> v[N1];
+    cx<T> X[M][N2];
+    WDM_UNROLL
+    for (int n1 = 0; n1 < N1; ++n1) v[n1] = ok ? row[N2 * n1 + m.j] : zero;
+    fwd2<F, DIR, P>(v, X, m.j, xb, tw);
+    if (ok) {
+        WDM_UNROLL
+        for (int i = 0; i < M; ++i)
+            WDM_UNROLL
+            for (int k2 = 0; k2 < N2; ++k2) row[(i * N2 + m.j) + N1 * k2] = X[i][k2];
+    }
+}
+
+// ------------------------------------------------------------------ reductions
+// Sum of k^2 |psi_hat|^2 (kinetic energy numerator) and sum |psi_hat|^2 over a
+// fully transformed grid; per-block partial pairs.
+template <class T>
+WDM_KERNEL(256)
+void k_spectral_sums(const cx<T>* __restrict__ ph, int nx, int ny, int nz, const double* __restrict__ k2x,
+                     const double* __restrict__ k2y, const double* __restrict__ k2z, double* __restrict__ partials) {
+    WDM_SHARED double s1[256], s2[256];
+    const int t = int(threadIdx.x);
+    const size_t n = size_t(nx) * ny * nz;
+    double a = 0.0, b = 0.0;
+    for (size_t i = size_t(blockIdx.x) * 256 + t; i < n; i += size_t(gridDim.x) * 256) {
+        const int x = int(i % nx);
+        const int y = int((i / nx) % ny);
+        const int z = int(i / (size_t(nx) * ny));
+        const double p = double(norm2(ph[i]));
+        a += (k2x[x] + k2y[y] + k2z[z]) * p;
+        b += p;
+    }
+    s1[t] = a;
+    s2[t] = b;
+    __syncthreads();
+    if (t == 0) {
+        double A = 0.0, B = 0.0;
+        for (int i = 0; i < 256; ++i) { A += s1[i]; B += s2[i]; }
+        partials[2 * blockIdx.x] = A;
+        partials[2 * blockIdx.x + 1] = B;
+    }
+}
+
+// sum rho*Phi and sum rho*V over the grid (W and V_ext numerators); rho from psi,
+// Phi from a real FP64 array.
+template <class T>
+WDM_KERNEL(256)
+void k_potential_sums(const cx<T>* __restrict__ psi, const double* __restrict__ phi, int nx, int ny, int nz,
+                      const double* __restrict__ xs, const double* __restrict__ ys, const double* __restrict__ zs,
+                      VModel vm, double* __restrict__ partials) {
+    WDM_SHARED double s1[256], s2[256];
+    const int t = int(threadIdx.x);
+    const size_t n = size_t(nx) * ny * nz;
+    double a = 0.0, b = 0.0;
+    for (size_t i = size_t(blockIdx.x) * 256 + t; i < n; i += size_t(gridDim.x) * 256) {
+        const int x = int(i % nx);
+        const int y = int((i / nx) % ny);
+        const int z = int(i / (size_t(nx) * ny));
+        const double r = double(norm2(psi[i]));
+        if (phi) a += r * phi[i];
+        if (vm.flags) b += r * eval_V<T>(vm, xs[x], ys[y], zs[z], unsigned(i), psi[i]);
+    }
+    s1[t] = a;
+    s2[t] = b;
+    __syncthreads();
+    if (t == 0) {
+        double A = 0.0, B = 0.0;
+        for (int i = 0; i < 256; ++i) { A += s1[i]; B += s2[i]; }
+        partials[2 * blockIdx.x] = A;
+        partials[2 * blockIdx.x + 1] = B;
+    }
+}
+
+// Element-wise conversions between the plan precision and FP64/FP32 host layouts.
+template <class TD, class TS>
+WDM_KERNEL(256)
+void k_convert(cx<TD>* __restrict__ dst, const cx<TS>* __restrict__ src, size_t n) {
+    for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += size_t(gridDim.x) * 256)
+        dst[i] = mk(TD(src[i].x), TD(src[i].y));
+}
+
+}  // namespace wdm

@@ -1,0 +1,225 @@
+// Complex arithmetic, compile-time twiddle constants and the precision policies
+// used by every pass kernel.
+//
+// Precision policies:
+//   Plain - ordinary multiplies by rounded constants.
+//   Comp  - (FP32 only) every multiply by a constant or twiddle uses the
+//           constant split as hi+lo floats (fma(x,hi,x*lo)), so the effective
+//           constant carries ~48 bits.  A constant rounded once to FP32 and
+//           applied to every cell every step would bias |psi|^2 by up to
+//           ~6e-8 per use; the split removes that bias, leaving only unbiased
+//           round-to-nearest errors in the data.
+#pragma once
+
+#include "wdm_port.hpp"
+
+namespace wdm {
+
+template <class T>
+struct alignas(2 * sizeof(T)) cx {
+    T x, y;
+};
+
+template <class T> WDM_HDFI cx<T> mk(T a, T b) { cx<T> r; r.x = a; r.y = b; return r; }
+template <class T> WDM_HDFI cx<T> operator+(cx<T> a, cx<T> b) { return mk(a.x + b.x, a.y + b.y); }
+template <class T> WDM_HDFI cx<T> operator-(cx<T> a, cx<T> b) { return mk(a.x - b.x, a.y - b.y); }
+template <class T> WDM_HDFI cx<T> operator-(cx<T> a) { return mk(-a.x, -a.y); }
+template <class T> WDM_HDFI cx<T> operator*(cx<T> a, T s) { return mk(a.x * s, a.y * s); }
+// multiply by i*s where s = +1 or -1 (exact)
+template <int S, class T> WDM_HDFI cx<T> mul_i(cx<T> a) {
+    if constexpr (S > 0) return mk(-a.y, a.x);
+    else return mk(a.y, -a.x);
+}
+template <class T> WDM_HDFI T norm2(cx<T> a) { return a.x * a.x + a.y * a.y; }
+
+WDM_HDFI float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+WDM_HDFI double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+struct Plain {};
+struct Comp {};
+
+// ---------------------------------------------------------------- constexpr trig
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+constexpr double kRsqrt2 = 0.70710678118654752440084436210485;
+
+constexpr double taylor_sin(double t) {
+    double t2 = t * t, term = t, s = t;
+    for (int n = 1; n < 14; ++n) {
+        term *= -t2 / double((2 * n) * (2 * n + 1));
+        s += term;
+    }
+    return s;
+}
+constexpr double taylor_cos(double t) {
+    double t2 = t * t, term = 1.0, c = 1.0;
+    for (int n = 1; n < 14; ++n) {
+        term *= -t2 / double((2 * n - 1) * (2 * n));
+        c += term;
+    }
+    return c;
+}
+
+struct CS {
+    double c, s;
+};
+
+// cos and sin of 2*pi*num/den, reduced to the first octant with exact
+// symmetries so that multiples of pi/4 come out exact.
+constexpr CS cs2pi(long num, long den) {
+    long n = ((num % den) + den) % den;
+    if ((8 * n) % den == 0) {
+        const double r = kRsqrt2;
+        switch ((8 * n) / den) {
+            case 0: return {1.0, 0.0};
+            case 1: return {r, r};
+            case 2: return {0.0, 1.0};
+            case 3: return {-r, r};
+            case 4: return {-1.0, 0.0};
+            case 5: return {-r, -r};
+            case 6: return {0.0, -1.0};
+            default: return {r, -r};
+        }
+    }
+    bool neg_s = false, neg_c = false, swap = false;
+    long d = den;
+    if (2 * n > d) { n = d - n; neg_s = true; }            // x -> 1 - x
+    if (4 * n > d) { n = d - 2 * n; d = 2 * d; neg_c = true; }  // x -> 1/2 - x
+    if (8 * n > d) { n = d - 4 * n; d = 4 * d; swap = true; }   // x -> 1/4 - x
+    double t = kTwoPi * double(n) / double(d);
+    double c = taylor_cos(t), s = taylor_sin(t);
+    if (swap) { double tmp = c; c = s; s = tmp; }
+    if (neg_c) c = -c;
+    if (neg_s) s = -s;
+    return {c, s};
+}
+
+template <int R>
+struct TwTab {
+    double c[R];
+    double s[R];
+    constexpr TwTab() : c(), s() {
+        for (int j = 0; j < R; ++j) {
+            CS v = cs2pi(j, R);
+            c[j] = v.c;
+            s[j] = v.s;
+        }
+    }
+};
+template <int R>
+inline constexpr TwTab<R> kTw{};
+
+// ---------------------------------------------------------------- constant multiplies
+// x * k for a real constant k (given in FP64).
+template <class P, class T> WDM_FI T kmul(T x, double k);
+template <> WDM_FI double kmul<Plain, double>(double x, double k) { return x * k; }
+template <> WDM_FI double kmul<Comp, double>(double x, double k) { return x * k; }
+template <> WDM_FI float kmul<Plain, float>(float x, double k) { return x * float(k); }
+template <> WDM_FI float kmul<Comp, float>(float x, double k) {
+    const float h = float(k);
+    const float l = float(k - double(h));
+    if (l == 0.0f) return x * h;
+    return fmaT(x, h, x * l);
+}
+template <class P, class T> WDM_FI cx<T> kmulc(cx<T> x, double k) {
+    return mk(kmul<P, T>(x.x, k), kmul<P, T>(x.y, k));
+}
+
+// x * (c + i s) for FP64 constants c, s.
+template <class P, class T> WDM_FI cx<T> cmulk(cx<T> a, double c, double s) {
+    if constexpr (std::is_same<T, double>::value || std::is_same<P, Plain>::value) {
+        const T cc = T(c), ss = T(s);
+        return mk(fmaT(a.x, cc, -(a.y * ss)), fmaT(a.x, ss, a.y * cc));
+    } else {
+        const float ch = float(c), cl = float(c - double(ch));
+        const float sh = float(s), sl = float(s - double(sh));
+        float t = a.x * cl;
+        t = fmaT(-a.y, sl, t);
+        t = fmaT(-a.y, sh, t);
+        t = fmaT(a.x, ch, t);
+        float u = a.x * sl;
+        u = fmaT(a.y, cl, u);
+        u = fmaT(a.y, ch, u);
+        u = fmaT(a.x, sh, u);
+        return mk(t, u);
+    }
+}
+
+// x * exp(DIR * 2 pi i j / R) with compile-time R, DIR and (after unrolling)
+// constant j; trivial angles become exact swaps/negations.
+template <int R, int DIR, class P, class T>
+WDM_FI cx<T> mul_w(cx<T> x, int j) {
+    if (j == 0) return x;
+    if ((4 * j) % R == 0) {
+        const int q = (4 * j) / R;
+        if (q == 2) return -x;
+        if ((q == 1) == (DIR > 0)) return mul_i<+1>(x);
+        return mul_i<-1>(x);
+    }
+    if ((8 * j) % R == 0) {
+        const int o = (8 * j) / R;  // 1,3,5,7
+        const bool cpos = (o == 1 || o == 7);
+        const bool spos = ((o == 1 || o == 3) == (DIR > 0));
+        const T re = (cpos ? x.x : -x.x) + (spos ? -x.y : x.y);
+        const T im = (spos ? x.x : -x.x) + (cpos ? x.y : -x.y);
+        return mk(kmul<P, T>(re, kRsqrt2), kmul<P, T>(im, kRsqrt2));
+    }
+    const double c = kTw<R>.c[j];
+    const double s = DIR > 0 ? kTw<R>.s[j] : -kTw<R>.s[j];
+    return cmulk<P, T>(x, c, s);
+}
+
+// Runtime twiddle table entries: FP64 {c, s}; FP32 {c_hi, s_hi, c_lo, s_lo}.
+struct TwF {
+    float ch, sh, cl, sl;
+};
+template <class T> struct TwEntry;
+template <> struct TwEntry<double> { typedef cx<double> type; };
+template <> struct TwEntry<float> { typedef TwF type; };
+
+// x * (table entry)^DIR-conjugated: forward (DIR<0) uses exp(-i theta).
+template <int DIR, class P>
+WDM_FI cx<double> mul_tab(cx<double> a, const cx<double>& w) {
+    const double s = DIR > 0 ? w.y : -w.y;
+    return mk(fmaT(a.x, w.x, -(a.y * s)), fmaT(a.x, s, a.y * w.x));
+}
+template <int DIR, class P>
+WDM_FI cx<float> mul_tab(cx<float> a, const TwF& w) {
+    const float sh = DIR > 0 ? w.sh : -w.sh;
+    if constexpr (std::is_same<P, Plain>::value) {
+        return mk(fmaT(a.x, w.ch, -(a.y * sh)), fmaT(a.x, sh, a.y * w.ch));
+    } else {
+        const float sl = DIR > 0 ? w.sl : -w.sl;
+        float t = a.x * w.cl;
+        t = fmaT(-a.y, sl, t);
+        t = fmaT(-a.y, sh, t);
+        t = fmaT(a.x, w.ch, t);
+        float u = a.x * sl;
+        u = fmaT(a.y, w.cl, u);
+        u = fmaT(a.y, w.ch, u);
+        u = fmaT(a.x, sh, u);
+        return mk(t, u);
+    }
+}
+
+// Multiply by an FP64 complex factor (c, s) computed at run time, in the data
+// precision: FP32 compensated splits it into hi+lo first.
+template <class P>
+WDM_FI cx<double> mul_dyn(cx<double> a, double c, double s) {
+    return mk(fmaT(a.x, c, -(a.y * s)), fmaT(a.x, s, a.y * c));
+}
+template <class P>
+WDM_FI cx<float> mul_dyn(cx<float> a, double c, double s) {
+    if constexpr (std::is_same<P, Plain>::value) {
+        const float cc = float(c), ss = float(s);
+        return mk(fmaT(a.x, cc, -(a.y * ss)), fmaT(a.x, ss, a.y * cc));
+    } else {
+        TwF w;
+        w.ch = float(c);
+        w.cl = float(c - double(w.ch));
+        w.sh = float(s);
+        w.sl = float(s - double(w.sh));
+        return mul_tab<+1, Comp>(a, w);
+    }
+}
+
+}  // namespace wdm

@@ -1,0 +1,163 @@
+// Per-axis-length launcher tables.  Each supported line length N has its own
+// translation unit (wdm_inst_<N>.cpp) that instantiates every pass kernel for
+// FP32 and FP64 and fills one AxisOps table per precision; the plan picks the
+// tables for Nx, Ny and Nz at creation time.
+#pragma once
+
+#include "wdm_kernels.hpp"
+
+#ifdef WDM_GPU
+#define WDM_LAUNCH(KERNEL, GRID, BLOCK, STREAM, ...) \
+    hipLaunchKernelGGL(KERNEL, GRID, BLOCK, 0, STREAM, __VA_ARGS__)
+#else
+#define WDM_LAUNCH(KERNEL, GRID, BLOCK, STREAM, ...) \
+    emu::launch(GRID, BLOCK, [&]() { KERNEL(__VA_ARGS__); })
+#endif
+
+namespace wdm {
+
+constexpr int kNW = 4;  // waves per workgroup for the pass kernels
+
+template <class T>
+struct AxisOps {
+    typedef typename TwEntry<T>::type TwE;
+    int n, n1, n2, lpw;
+    void (*col)(int mode, int kf, cx<T>* data, int ncols, int nother, unsigned sline, unsigned sother,
+                const TwE* tw, KTabs kt, hipStream_t s);
+    void (*x_kick)(bool fwd, cx<T>* psi, const cx<T>* phihat, int nlines, int hpitch, const TwE* tw,
+                   const KickArgs& ka, hipStream_t s);
+    void (*x_inv_dens)(cx<T>* psi, cx<T>* rhohat, int nlines, int hpitch, const TwE* tw, const double* border,
+                       double* partials, hipStream_t s);
+    void (*x_dens)(const cx<T>* psi, cx<T>* rhohat, int nlines, int hpitch, const TwE* tw, double* partials,
+                   hipStream_t s);
+    void (*x_r2c_real)(const double* src, cx<T>* rhohat, int nlines, int hpitch, const TwE* tw, hipStream_t s);
+    void (*x_c2r_real)(const cx<T>* h, double* dst, int nlines, int hpitch, const TwE* tw, hipStream_t s);
+    void (*x_fft)(int dir, cx<T>* psi, int nlines, const TwE* tw, hipStream_t s);
+};
+
+// Grid sizes (in workgroups of kNW waves).
+template <class F>
+inline unsigned x_blocks(int nlines) {
+    const int waves = (nlines + F::LPW - 1) / F::LPW;
+    return unsigned((waves + kNW - 1) / kNW);
+}
+template <class F>
+inline unsigned col_blocks(int ncols, int nother) {
+    const long waves = long((ncols + F::LPW - 1) / F::LPW) * nother;
+    return unsigned((waves + kNW - 1) / kNW);
+}
+
+template <class T, class F>
+struct Launch {
+    typedef typename TwEntry<T>::type TwE;
+    typedef typename std::conditional<std::is_same<T, float>::value, Comp, Plain>::type PsiP;
+
+    template <int MODE, int KF>
+    static void col_t(cx<T>* d, int ncols, int nother, unsigned sl, unsigned so, const TwE* tw, KTabs kt,
+                      hipStream_t s) {
+        WDM_LAUNCH((k_col<T, PsiP, F, MODE, KF, kNW>), dim3(col_blocks<F>(ncols, nother)), dim3(kNW * 64), s, d,
+                   ncols, nother, sl, so, tw, kt);
+    }
+    template <int MODE, int KF>
+    static void colp_t(cx<T>* d, int ncols, int nother, unsigned sl, unsigned so, const TwE* tw, KTabs kt,
+                       hipStream_t s) {
+        WDM_LAUNCH((k_col<T, Plain, F, MODE, KF, kNW>), dim3(col_blocks<F>(ncols, nother)), dim3(kNW * 64), s, d,
+                   ncols, nother, sl, so, tw, kt);
+    }
+    // mode/kf combinations used by the plan; KF_POISSON passes run in Plain.
+    static void col(int mode, int kf, cx<T>* d, int ncols, int nother, unsigned sl, unsigned so, const TwE* tw,
+                    KTabs kt, hipStream_t s) {
+        if (kf == KF_POISSON) {
+            colp_t<COL_FMI, KF_POISSON>(d, ncols, nother, sl, so, tw, kt, s);
+            return;
+        }
+        if (kf == KF_PNONE) {  // Poisson-chain y passes
+            if (mode == COL_FWD) colp_t<COL_FWD, KF_PNONE>(d, ncols, nother, sl, so, tw, kt, s);
+            else colp_t<COL_INV, KF_PNONE>(d, ncols, nother, sl, so, tw, kt, s);
+            return;
+        }
+        switch (mode) {
+            case COL_FWD: col_t<COL_FWD, KF_NONE>(d, ncols, nother, sl, so, tw, kt, s); break;
+            case COL_INV: col_t<COL_INV, KF_NONE>(d, ncols, nother, sl, so, tw, kt, s); break;
+            case COL_FMI: col_t<COL_FMI, KF_DRIFT>(d, ncols, nother, sl, so, tw, kt, s); break;
+            case COL_MI: col_t<COL_MI, KF_DRIFT>(d, ncols, nother, sl, so, tw, kt, s); break;
+        }
+    }
+    static void x_kick(bool fwd, cx<T>* psi, const cx<T>* ph, int nlines, int hp, const TwE* tw, const KickArgs& ka,
+                       hipStream_t s) {
+        if (fwd)
+            WDM_LAUNCH((k_x_kick<T, PsiP, F, true, kNW>), dim3(x_blocks<F>(nlines)), dim3(kNW * 64), s, psi, ph,
+                       nlines, hp, tw, ka);
+        else
+            WDM_LAUNCH((k_x_kick<T, PsiP, F, false, kNW>), dim3(x_blocks<F>(nlines)), dim3(kNW * 64), s, psi, ph,
+                       nlines, hp, tw, ka);
+    }
+    static void x_inv_dens(cx<T>* psi, cx<T>* rh, int nlines, int hp, const TwE* tw, const double* border,
+                           double* partials, hipStream_t s) {
+        WDM_LAUNCH((k_x_inv_dens<T, PsiP, F, kNW>), dim3(x_blocks<F>(nlines)), dim3(kNW * 64), s, psi, rh, nlines,
+                   hp, tw, border, partials);
+    }
+    static void x_dens(const cx<T>* psi, cx<T>* rh, int nlines, int hp, const TwE* tw, double* partials,
+                       hipStream_t s) {
+        WDM_LAUNCH((k_x_dens<T, F, kNW>), dim3(x_blocks<F>(nlines)), dim3(kNW * 64), s, psi, rh, nlines, hp, tw,
+                   partials);
+    }
+    static void x_r2c_real(const double* src, cx<T>* rh, int nlines, int hp, const TwE* tw, hipStream_t s) {
+        WDM_LAUNCH((k_x_r2c_real<T, F, kNW>), dim3(x_blocks<F>(nlines)), dim3(kNW * 64), s, src, rh, nlines, hp,
+                   tw);
+    }
+    static void x_c2r_real(const cx<T>* h, double* dst, int nlines, int hp, const TwE* tw, hipStream_t s) {
+        WDM_LAUNCH((k_x_c2r_real<T, F, kNW>), dim3(x_blocks<F>(nlines)), dim3(kNW * 64), s, h, dst, nlines, hp,
+                   tw);
+    }
+    static void x_fft(int dir, cx<T>* psi, int nlines, const TwE* tw, hipStream_t s) {
+        if (dir < 0)
+            WDM_LAUNCH((k_x_fft<T, PsiP, F, -1, kNW>), dim3(x_blocks<F>(nlines)), dim3(kNW * 64), s, psi, nlines,
+                       tw);
+        else
+            WDM_LAUNCH((k_x_fft<T, PsiP, F, +1, kNW>), dim3(x_blocks<F>(nlines)), dim3(kNW * 64), s, psi, nlines,
+                       tw);
+    }
+    static const AxisOps<T>* table() {
+        static const AxisOps<T> ops = {F::N, F::N1, F::N2, F::LPW, &col, &x_kick, &x_inv_dens,
+                                       &x_dens, &x_r2c_real, &x_c2r_real, &x_fft};
+        return &ops;
+    }
+};
+
+// Supported axis lengths and their (N1, N2) factorisations.
+#define WDM_FOR_EACH_SIZE(X) \
+    X(1, 1, 1)               \
+    X(8, 8, 1)               \
+    X(12, 12, 1)             \
+    X(16, 16, 1)             \
+    X(32, 8, 4)              \
+    X(64, 8, 8)              \
+    X(128, 16, 8)            \
+    X(160, 40, 4)            \
+    X(256, 32, 8)            \
+    X(384, 48, 8)
+
+#define WDM_DECLARE_SIZE(N, A, B)                   \
+    const AxisOps<float>* axis_ops_f32_##N();       \
+    const AxisOps<double>* axis_ops_f64_##N();
+WDM_FOR_EACH_SIZE(WDM_DECLARE_SIZE)
+
+#define WDM_DEFINE_SIZE(N, A, B)                                                           \
+    const AxisOps<float>* axis_ops_f32_##N() { return Launch<float, Fac<A, B>>::table(); } \
+    const AxisOps<double>* axis_ops_f64_##N() { return Launch<double, Fac<A, B>>::table(); }
+
+inline const AxisOps<float>* axis_ops(int n, float*) {
+#define WDM_CASE_F32(N, A, B) \
+    case N: return axis_ops_f32_##N();
+    switch (n) { WDM_FOR_EACH_SIZE(WDM_CASE_F32) default: return nullptr; }
+#undef WDM_CASE_F32
+}
+inline const AxisOps<double>* axis_ops(int n, double*) {
+#define WDM_CASE_F64(N, A, B) \
+    case N: return axis_ops_f64_##N();
+    switch (n) { WDM_FOR_EACH_SIZE(WDM_CASE_F64) default: return nullptr; }
+#undef WDM_CASE_F64
+}
+
+}  // namespace wdm

@@ -1,0 +1,1023 @@
+// Plan object and C ABI (include/wavedm.h).  Compiled as HIP for the product
+// library and as plain C++ for the CPU emulation test build.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <exception>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/wavedm.h"
+#include "wdm_ops.hpp"
+
+using namespace wdm;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct WdmError {
+    int code;
+    std::string msg;
+};
+
+[[noreturn]] void fail(int code, const std::string& msg) { throw WdmError{code, msg}; }
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) fail(WDM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class T>
+T* dalloc(size_t n) {
+    void* p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(&p, n * sizeof(T));
+    if (e != hipSuccess) fail(WDM_ERR_OOM, "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed");
+    return static_cast<T*>(p);
+}
+
+struct DBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void reset() {
+        if (p) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T>
+    T* get(size_t n) {
+        const size_t b = std::max<size_t>(n * sizeof(T), 1);
+        if (bytes < b) {
+            reset();
+            p = dalloc<char>(b);
+            bytes = b;
+        }
+        return static_cast<T*>(p);
+    }
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+// Julia LinRange(a, b, n)[j] = (1 - t) a + t b, t = j/(n-1)
+double linrange_at(double a, double b, int n, int j) {
+    if (n == 1) return a;
+    const double t = double(j) / double(n - 1);
+    return (1.0 - t) * a + t * b;
+}
+
+// drift-lattice frequency of FFT index m along one axis (src/solver/fft_operators.jl:9-13):
+// f = LinRange(-n/(4X), n/(4X) - 1/(2X), n)[(m - n div 2) mod n]; k = 2 pi f.
+double drift_k(int n, double X, int m) {
+    if (n == 1) return 0.0;
+    const int j = ((m - n / 2) % n + n) % n;
+    const double f = linrange_at(-n / 4.0 / X, n / 4.0 / X - 1.0 / 2.0 / X, n, j);
+    return 2.0 * M_PI * f;
+}
+
+int fft_int(int n, int m) { return (2 * m < n) ? m : m - n; }
+
+}  // namespace
+
+struct AxisState {
+    int n = 0;
+    const void* ops = nullptr;  // AxisOps<T>*
+    DBuf tw;                    // twiddle table
+    DBuf coord;                 // FP64 coordinates
+    DBuf drift;                 // complex FP64 drift factors for dt_cached
+    DBuf k2;                    // FP64 k^2 (FFT order; x axis: half spectrum)
+};
+
+struct wdm_plan {
+    int nx, ny, nz;
+    size_t ncell;
+    double X[3];
+    double dx[3];
+    int precision;
+    unsigned flags;
+    double hbar_over_m = 1.0;
+    double four_pi_G = 4.0 * M_PI;
+    bool self_gravity = true;
+    int hpitch = 0;
+    hipStream_t stream = nullptr;
+    AxisState ax[3];
+    DBuf psi, rhohat, partials, redout, border, vstatic, scratch, scratch_real;
+    size_t npartials = 0;
+    bool border_on = false;
+    // state
+    bool kspace = false;    // psi holds the 3-D spectrum (after wdm_kick)
+    bool phi_valid = false; // rhohat holds Phi_hat of the current psi
+    bool mass_valid = false;
+    double dt_cached = NAN, hbar_cached = NAN;
+    // potential models
+    int vflags = 0;
+    double hq[5] = {0, 0, 0, 0, 0};
+    double quad[13] = {0};
+    double kappa = 0.0;
+    double last_kick_t = 0.0;
+    std::string err;
+};
+
+namespace {
+
+template <class T>
+const AxisOps<T>* ops_of(const wdm_plan* p, int a) {
+    return static_cast<const AxisOps<T>*>(p->ax[a].ops);
+}
+
+template <class T>
+void build_twiddles(AxisState& a) {
+    typedef typename TwEntry<T>::type TwE;
+    std::vector<TwE> h(a.n);
+    for (int j = 0; j < a.n; ++j) {
+        const CS v = cs2pi(j, a.n);
+        if constexpr (std::is_same<T, double>::value) {
+            h[j] = mk(v.c, v.s);
+        } else {
+            TwF w;
+            w.ch = float(v.c);
+            w.cl = float(v.c - double(w.ch));
+            w.sh = float(v.s);
+            w.sl = float(v.s - double(w.sh));
+            h[j] = w;
+        }
+    }
+    TwE* d = a.tw.get<TwE>(a.n);
+    hip_check(hipMemcpy(d, h.data(), sizeof(TwE) * a.n, hipMemcpyHostToDevice), "twiddle upload");
+}
+
+void upload(DBuf& b, const std::vector<double>& h) {
+    // synchronous copy; callers replacing a table that queued kernels may
+    // still read synchronise the plan stream first
+    double* d = b.get<double>(h.size());
+    hip_check(hipMemcpy(d, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice), "table upload");
+}
+
+void ensure_drift(wdm_plan* p, double dt) {
+    if (p->dt_cached == dt && p->hbar_cached == p->hbar_over_m) return;
+    hip_check(hipStreamSynchronize(p->stream), "drift table sync");
+    for (int a = 0; a < 3; ++a) {
+        const int n = p->ax[a].n;
+        std::vector<double> h(2 * n);
+        for (int m = 0; m < n; ++m) {
+            const double k = drift_k(n, p->X[a], m);
+            const double th = p->hbar_over_m * k * k * dt / 2.0;
+            h[2 * m] = std::cos(th);
+            h[2 * m + 1] = -std::sin(th);
+        }
+        upload(p->ax[a].drift, h);
+    }
+    p->dt_cached = dt;
+    p->hbar_cached = p->hbar_over_m;
+}
+
+void build_k2(wdm_plan* p) {
+    for (int a = 0; a < 3; ++a) {
+        const int n = p->ax[a].n;
+        const int cnt = (a == 0) ? n / 2 + 1 : n;
+        const double L = 2.0 * p->X[a];
+        std::vector<double> h(cnt);
+        for (int m = 0; m < cnt; ++m) {
+            const double k = 2.0 * M_PI * (a == 0 ? m : fft_int(n, m)) / L;
+            h[m] = k * k;
+        }
+        upload(p->ax[a].k2, h);
+    }
+}
+
+template <class T>
+void launch_check(wdm_plan* p, const char* what) {
+    (void)p;
+    hip_check(hipGetLastError(), what);
+}
+
+// ---- pass sequences ------------------------------------------------------
+template <class T>
+void poisson_chain(wdm_plan* p, double scale) {
+    // rhohat holds R2C_x(rho); run y FWD, z (FWD * scale/k^2 * INV), y INV.
+    const AxisOps<T>* oy = ops_of<T>(p, 1);
+    const AxisOps<T>* oz = ops_of<T>(p, 2);
+    typedef typename TwEntry<T>::type TwE;
+    cx<T>* rh = p->rhohat.as<cx<T>>();
+    const int nh = p->nx / 2 + 1, hp = p->hpitch;
+    KTabs none = {nullptr, nullptr, nullptr, 0.0};
+    if (p->ny > 1)
+        oy->col(COL_FWD, KF_PNONE, rh, nh, p->nz, unsigned(hp), unsigned(hp * p->ny), p->ax[1].tw.as<TwE>(), none,
+                p->stream);
+    // Phi_k = -scale * rho_k / k^2, with the 1/N of the unnormalised inverse folded in
+    KTabs kt = {p->ax[0].k2.as<double>(), p->ax[1].k2.as<double>(), p->ax[2].k2.as<double>(),
+                -scale / double(p->ncell)};
+    oz->col(COL_FMI, KF_POISSON, rh, nh, p->ny, unsigned(hp * p->ny), unsigned(hp), p->ax[2].tw.as<TwE>(), kt,
+            p->stream);
+    if (p->ny > 1)
+        oy->col(COL_INV, KF_PNONE, rh, nh, p->nz, unsigned(hp), unsigned(hp * p->ny), p->ax[1].tw.as<TwE>(), none,
+                p->stream);
+    launch_check<T>(p, "poisson chain");
+}
+
+template <class T>
+void compute_phi(wdm_plan* p) {
+    typedef typename TwEntry<T>::type TwE;
+    const AxisOps<T>* ox = ops_of<T>(p, 0);
+    const int nlines = p->ny * p->nz;
+    ox->x_dens(p->psi.as<cx<T>>(), p->rhohat.as<cx<T>>(), nlines, p->hpitch, p->ax[0].tw.as<TwE>(),
+               p->partials.as<double>(), p->stream);
+    hip_check(hipGetLastError(), "density pass");
+    poisson_chain<T>(p, p->four_pi_G);
+    p->phi_valid = true;
+    p->mass_valid = true;
+}
+
+void rotation_matrix(const double* n_in, double ang, double R[3][3]) {
+    double n[3] = {n_in[0], n_in[1], n_in[2]};
+    const double nn = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    if (nn == 0.0) {
+        n[0] = 0; n[1] = 0; n[2] = 1;
+    } else {
+        for (double& v : n) v /= nn;
+    }
+    const double c = std::cos(ang), s = std::sin(ang), C = 1.0 - c;
+    R[0][0] = c + n[0] * n[0] * C;
+    R[0][1] = n[0] * n[1] * C - n[2] * s;
+    R[0][2] = n[0] * n[2] * C + n[1] * s;
+    R[1][0] = n[1] * n[0] * C + n[2] * s;
+    R[1][1] = c + n[1] * n[1] * C;
+    R[1][2] = n[1] * n[2] * C - n[0] * s;
+    R[2][0] = n[2] * n[0] * C - n[1] * s;
+    R[2][1] = n[2] * n[1] * C + n[0] * s;
+    R[2][2] = c + n[2] * n[2] * C;
+}
+
+VModel make_vmodel(const wdm_plan* p, double t) {
+    VModel vm;
+    std::memset(&vm, 0, sizeof(vm));
+    vm.flags = p->vflags;
+    vm.vstatic = p->vstatic.as<double>();
+    const double G = p->four_pi_G / (4.0 * M_PI);
+    vm.hq_GM = G * p->hq[0];
+    vm.hq_a = p->hq[1];
+    vm.hq_c[0] = p->hq[2];
+    vm.hq_c[1] = p->hq[3];
+    vm.hq_c[2] = p->hq[4];
+    if (p->vflags & V_QUADRUPOLE) {
+        const double* q = p->quad;
+        const double T0[3][3] = {{q[0], q[3], q[4]}, {q[3], q[1], q[5]}, {q[4], q[5], q[2]}};
+        const double period = q[9];
+        const double ang = period != 0.0 ? 2.0 * M_PI * t / period : 0.0;
+        double R[3][3];
+        rotation_matrix(q + 6, ang, R);
+        double RT[3][3] = {{0}}, Tt[3][3] = {{0}};
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                for (int k = 0; k < 3; ++k) RT[i][j] += R[i][k] * T0[k][j];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                for (int k = 0; k < 3; ++k) Tt[i][j] += RT[i][k] * R[j][k];
+        vm.qT[0] = Tt[0][0];
+        vm.qT[1] = Tt[1][1];
+        vm.qT[2] = Tt[2][2];
+        vm.qT[3] = Tt[0][1];
+        vm.qT[4] = Tt[0][2];
+        vm.qT[5] = Tt[1][2];
+        vm.q_c[0] = q[10];
+        vm.q_c[1] = q[11];
+        vm.q_c[2] = q[12];
+    }
+    vm.kappa = p->kappa;
+    return vm;
+}
+
+template <class T>
+KickArgs make_kick(const wdm_plan* p, double dt_kick, double t) {
+    KickArgs ka;
+    ka.mh_dt = dt_kick / p->hbar_over_m;
+    ka.t = t;
+    ka.use_phi = p->self_gravity ? 1 : 0;
+    ka.Ny = p->ny;
+    ka.xs = p->ax[0].coord.as<double>();
+    ka.ys = p->ax[1].coord.as<double>();
+    ka.zs = p->ax[2].coord.as<double>();
+    ka.vm = make_vmodel(p, t);
+    return ka;
+}
+
+template <class T>
+void ensure_phi(wdm_plan* p) {
+    if (p->self_gravity && !p->phi_valid) compute_phi<T>(p);
+}
+
+// kick x-pass (FWD: fused forward x-FFT)
+template <class T>
+void x_kick(wdm_plan* p, bool fwd, double dt_kick, double t) {
+    typedef typename TwEntry<T>::type TwE;
+    const AxisOps<T>* ox = ops_of<T>(p, 0);
+    const KickArgs ka = make_kick<T>(p, dt_kick, t);
+    ox->x_kick(fwd, p->psi.as<cx<T>>(), p->rhohat.as<cx<T>>(), p->ny * p->nz, p->hpitch, p->ax[0].tw.as<TwE>(), ka,
+               p->stream);
+    hip_check(hipGetLastError(), "kick pass");
+    p->last_kick_t = t;
+    p->phi_valid = false;
+    p->mass_valid = false;
+}
+
+template <class T>
+void y_pass(wdm_plan* p, int mode) {
+    typedef typename TwEntry<T>::type TwE;
+    if (p->ny == 1) return;
+    const AxisOps<T>* oy = ops_of<T>(p, 1);
+    KTabs none = {nullptr, nullptr, nullptr, 0.0};
+    oy->col(mode, KF_NONE, p->psi.as<cx<T>>(), p->nx, p->nz, unsigned(p->nx), unsigned(p->nx * p->ny),
+            p->ax[1].tw.as<TwE>(), none, p->stream);
+    hip_check(hipGetLastError(), "y pass");
+}
+
+template <class T>
+void z_pass(wdm_plan* p, int mode, double dt) {
+    typedef typename TwEntry<T>::type TwE;
+    const AxisOps<T>* oz = ops_of<T>(p, 2);
+    KTabs kt = {nullptr, nullptr, nullptr, 0.0};
+    if (mode == COL_FMI || mode == COL_MI) {
+        ensure_drift(p, dt);
+        kt = {p->ax[0].drift.as<double>(), p->ax[1].drift.as<double>(), p->ax[2].drift.as<double>(),
+              1.0 / double(p->ncell)};
+    } else if (p->nz == 1) {
+        return;
+    }
+    oz->col(mode, KF_NONE, p->psi.as<cx<T>>(), p->nx, p->ny, unsigned(p->nx * p->ny), unsigned(p->nx),
+            p->ax[2].tw.as<TwE>(), kt, p->stream);
+    hip_check(hipGetLastError(), "z pass");
+}
+
+// inverse x-pass + border + density (+ Poisson chain when self-gravity is on)
+template <class T>
+void x_inv_close(wdm_plan* p) {
+    typedef typename TwEntry<T>::type TwE;
+    const AxisOps<T>* ox = ops_of<T>(p, 0);
+    ox->x_inv_dens(p->psi.as<cx<T>>(), p->self_gravity ? p->rhohat.as<cx<T>>() : nullptr, p->ny * p->nz,
+                   p->hpitch, p->ax[0].tw.as<TwE>(), p->border_on ? p->border.as<double>() : nullptr,
+                   p->partials.as<double>(), p->stream);
+    hip_check(hipGetLastError(), "inverse x pass");
+    p->kspace = false;
+    p->mass_valid = p->self_gravity;
+    if (p->self_gravity) {
+        poisson_chain<T>(p, p->four_pi_G);
+        p->phi_valid = true;
+    }
+}
+
+template <class T>
+void do_kick(wdm_plan* p, double dt_kick, double t) {
+    if (p->kspace) fail(WDM_ERR_STATE, "wdm_kick: psi is in k-space (call wdm_drift first)");
+    ensure_phi<T>(p);
+    x_kick<T>(p, true, dt_kick, t);
+    y_pass<T>(p, COL_FWD);
+    z_pass<T>(p, COL_FWD, 0.0);
+    p->kspace = true;
+}
+
+template <class T>
+void do_drift(wdm_plan* p, double dt) {
+    if (!p->kspace) fail(WDM_ERR_STATE, "wdm_drift: psi is not in k-space (call wdm_kick first)");
+    z_pass<T>(p, COL_MI, dt);
+    y_pass<T>(p, COL_INV);
+    x_inv_close<T>(p);
+}
+
+template <class T>
+void do_step(wdm_plan* p, int nsteps, double dt, double dt_kick, double t0, int split) {
+    if (p->kspace) fail(WDM_ERR_STATE, "wdm_step: psi is in k-space (call wdm_drift first)");
+    if (nsteps <= 0) return;
+    ensure_phi<T>(p);
+    ensure_drift(p, dt);
+    if (split == WDM_SPLIT_REFERENCE) {
+        for (int s = 0; s < nsteps; ++s) {
+            x_kick<T>(p, true, dt_kick, t0 + s * dt);
+            y_pass<T>(p, COL_FWD);
+            z_pass<T>(p, COL_FMI, dt);
+            y_pass<T>(p, COL_INV);
+            x_inv_close<T>(p);
+        }
+    } else {
+        typedef typename TwEntry<T>::type TwE;
+        const AxisOps<T>* ox = ops_of<T>(p, 0);
+        x_kick<T>(p, false, 0.5 * dt, t0);
+        ox->x_fft(-1, p->psi.as<cx<T>>(), p->ny * p->nz, p->ax[0].tw.as<TwE>(), p->stream);
+        for (int s = 0; s < nsteps; ++s) {
+            y_pass<T>(p, COL_FWD);
+            z_pass<T>(p, COL_FMI, dt);
+            y_pass<T>(p, COL_INV);
+            x_inv_close<T>(p);
+            const double t = t0 + (s + 1) * dt;
+            if (s + 1 < nsteps) x_kick<T>(p, true, dt, t);
+            else x_kick<T>(p, false, 0.5 * dt, t);
+        }
+    }
+}
+
+template <class T>
+void forward3d(wdm_plan* p, cx<T>* buf) {
+    typedef typename TwEntry<T>::type TwE;
+    const AxisOps<T>* ox = ops_of<T>(p, 0);
+    const AxisOps<T>* oy = ops_of<T>(p, 1);
+    const AxisOps<T>* oz = ops_of<T>(p, 2);
+    KTabs none = {nullptr, nullptr, nullptr, 0.0};
+    ox->x_fft(-1, buf, p->ny * p->nz, p->ax[0].tw.as<TwE>(), p->stream);
+    if (p->ny > 1)
+        oy->col(COL_FWD, KF_NONE, buf, p->nx, p->nz, unsigned(p->nx), unsigned(p->nx * p->ny),
+                p->ax[1].tw.as<TwE>(), none, p->stream);
+    if (p->nz > 1)
+        oz->col(COL_FWD, KF_NONE, buf, p->nx, p->ny, unsigned(p->nx * p->ny), unsigned(p->nx),
+                p->ax[2].tw.as<TwE>(), none, p->stream);
+    hip_check(hipGetLastError(), "forward 3-D FFT");
+}
+
+template <class T>
+void phi_real(wdm_plan* p, double* dst_dev) {
+    typedef typename TwEntry<T>::type TwE;
+    ensure_phi<T>(p);
+    if (!p->self_gravity) {
+        hip_check(hipMemsetAsync(dst_dev, 0, sizeof(double) * p->ncell, p->stream), "memset phi");
+        return;
+    }
+    const AxisOps<T>* ox = ops_of<T>(p, 0);
+    ox->x_c2r_real(p->rhohat.as<cx<T>>(), dst_dev, p->ny * p->nz, p->hpitch, p->ax[0].tw.as<TwE>(), p->stream);
+    hip_check(hipGetLastError(), "C2R pass");
+}
+
+// Final reduction of per-wave partial pairs (sum, max): Neumaier-compensated
+// sum in a fixed order, deterministic.  One block of 256 threads.
+WDM_KERNEL(256)
+void k_reduce_partials(const double* __restrict__ partials, int n, int second_sum, double* __restrict__ out) {
+    WDM_SHARED double ssum[256], scmp[256], smax[256];
+    const int t = int(threadIdx.x);
+    double s = 0.0, comp = 0.0, mx = 0.0;
+    for (int i = t; i < n; i += 256) {
+        const double v = partials[2 * i];
+        const double tt = s + v;
+        if (__builtin_fabs(s) >= __builtin_fabs(v)) comp += (s - tt) + v;
+        else comp += (v - tt) + s;
+        s = tt;
+        const double mv = partials[2 * i + 1];
+        mx = second_sum ? mx + mv : (mv > mx ? mv : mx);
+    }
+    ssum[t] = s;
+    scmp[t] = comp;
+    smax[t] = mx;
+    __syncthreads();
+    if (t == 0) {
+        double S = 0.0, C = 0.0, MX = 0.0;
+        for (int i = 0; i < 256; ++i) {
+            const double v = ssum[i];
+            const double tt = S + v;
+            if (__builtin_fabs(S) >= __builtin_fabs(v)) C += (S - tt) + v;
+            else C += (v - tt) + S;
+            S = tt;
+            C += scmp[i];
+            MX = second_sum ? MX + smax[i] : (smax[i] > MX ? smax[i] : MX);
+        }
+        out[0] = S + C;
+        out[1] = MX;
+    }
+}
+
+unsigned red_blocks(size_t n) { return unsigned(std::min<size_t>((n + 255) / 256, 1024)); }
+
+// generic grid reductions ---------------------------------------------------
+WDM_KERNEL(256)
+void k_abs_max_sum(const double* __restrict__ a, size_t n, double* __restrict__ partials) {
+    WDM_SHARED double s1[256], s2[256];
+    const int t = int(threadIdx.x);
+    double s = 0.0, mx = 0.0;
+    for (size_t i = size_t(blockIdx.x) * 256 + t; i < n; i += size_t(gridDim.x) * 256) {
+        const double v = a[i];
+        s += v;
+        const double av = __builtin_fabs(v);
+        mx = av > mx ? av : mx;
+    }
+    s1[t] = s;
+    s2[t] = mx;
+    __syncthreads();
+    if (t == 0) {
+        double S = 0.0, MX = 0.0;
+        for (int i = 0; i < 256; ++i) {
+            S += s1[i];
+            MX = s2[i] > MX ? s2[i] : MX;
+        }
+        partials[2 * blockIdx.x] = S;
+        partials[2 * blockIdx.x + 1] = MX;
+    }
+}
+
+template <class T>
+WDM_KERNEL(256)
+void k_psi_sums(const cx<T>* __restrict__ psi, size_t n, double* __restrict__ partials) {
+    WDM_SHARED double s1[256], s2[256];
+    const int t = int(threadIdx.x);
+    double s = 0.0, mx = 0.0;
+    for (size_t i = size_t(blockIdx.x) * 256 + t; i < n; i += size_t(gridDim.x) * 256) {
+        const double v = double(norm2(psi[i]));
+        s += v;
+        mx = v > mx ? v : mx;
+    }
+    s1[t] = s;
+    s2[t] = mx;
+    __syncthreads();
+    if (t == 0) {
+        double S = 0.0, MX = 0.0;
+        for (int i = 0; i < 256; ++i) {
+            S += s1[i];
+            MX = s2[i] > MX ? s2[i] : MX;
+        }
+        partials[2 * blockIdx.x] = S;
+        partials[2 * blockIdx.x + 1] = MX;
+    }
+}
+
+// out[0] = compensated sum of the first slots; out[1] = max (or sum) of the second
+void finalize(wdm_plan* p, int n, double out[2], bool second_sum = false) {
+    double* o = p->redout.get<double>(2);
+    WDM_LAUNCH((k_reduce_partials), dim3(1), dim3(256), p->stream, p->partials.as<double>(), n,
+               second_sum ? 1 : 0, o);
+    hip_check(hipGetLastError(), "reduce");
+    hip_check(hipMemcpyAsync(out, o, 2 * sizeof(double), hipMemcpyDeviceToHost, p->stream), "reduce copy");
+    hip_check(hipStreamSynchronize(p->stream), "reduce sync");
+}
+
+template <class T>
+void psi_sums(wdm_plan* p, double out[2]) {
+    const unsigned nb = red_blocks(p->ncell);
+    p->partials.get<double>(std::max<size_t>(p->npartials, 2 * nb));
+    WDM_LAUNCH((k_psi_sums<T>), dim3(nb), dim3(256), p->stream, p->psi.as<cx<T>>(), p->ncell,
+               p->partials.as<double>());
+    hip_check(hipGetLastError(), "psi sums");
+    finalize(p, int(nb), out);
+}
+
+template <class T>
+void reduce_all(wdm_plan* p, double* out, int nout) {
+    if (p->kspace) fail(WDM_ERR_STATE, "wdm_reduce: psi is in k-space");
+    double r[WDM_RED_COUNT] = {0};
+    const double dV = p->dx[0] * p->dx[1] * p->dx[2];
+    double ms[2];
+    psi_sums<T>(p, ms);
+    r[WDM_RED_MASS] = ms[0] * dV;
+    r[WDM_RED_PSIMAX] = std::sqrt(ms[1]);
+    if (nout > WDM_RED_KINETIC) {
+        cx<T>* sc = p->scratch.get<cx<T>>(p->ncell);
+        hip_check(hipMemcpyAsync(sc, p->psi.as<cx<T>>(), sizeof(cx<T>) * p->ncell, hipMemcpyDeviceToDevice,
+                                 p->stream),
+                  "copy");
+        forward3d<T>(p, sc);
+        // full-length k^2 tables (FFT order on every axis)
+        std::vector<double> hk[3];
+        DBuf kfull[3];
+        for (int a = 0; a < 3; ++a) {
+            const int n = p->ax[a].n;
+            hk[a].resize(n);
+            for (int m = 0; m < n; ++m) {
+                const double k = 2.0 * M_PI * fft_int(n, m) / (2.0 * p->X[a]);
+                hk[a][m] = k * k;
+            }
+            upload(kfull[a], hk[a]);
+        }
+        const unsigned nb = red_blocks(p->ncell);
+        p->partials.get<double>(std::max<size_t>(p->npartials, 2 * nb));
+        WDM_LAUNCH((k_spectral_sums<T>), dim3(nb), dim3(256), p->stream, sc, p->nx, p->ny, p->nz,
+                   kfull[0].as<double>(), kfull[1].as<double>(), kfull[2].as<double>(), p->partials.as<double>());
+        hip_check(hipGetLastError(), "spectral sums");
+        double ks[2];
+        finalize(p, int(nb), ks, true);
+        for (auto& b : kfull) b.reset();
+        r[WDM_RED_KINETIC] = 0.5 * p->hbar_over_m * p->hbar_over_m * ks[0] * dV / double(p->ncell);
+    }
+    if (nout > WDM_RED_WSELF) {
+        double* phi = p->scratch_real.get<double>(p->ncell);
+        const bool sg = p->self_gravity;
+        if (sg) phi_real<T>(p, phi);
+        const unsigned nb = red_blocks(p->ncell);
+        p->partials.get<double>(std::max<size_t>(p->npartials, 2 * nb));
+        VModel vm = make_vmodel(p, p->last_kick_t);
+        WDM_LAUNCH((k_potential_sums<T>), dim3(nb), dim3(256), p->stream, p->psi.as<cx<T>>(),
+                   sg ? (const double*)phi : nullptr, p->nx, p->ny, p->nz, p->ax[0].coord.as<double>(),
+                   p->ax[1].coord.as<double>(), p->ax[2].coord.as<double>(), vm, p->partials.as<double>());
+        hip_check(hipGetLastError(), "potential sums");
+        double ws[2];
+        finalize(p, int(nb), ws, true);
+        r[WDM_RED_WSELF] = 0.5 * ws[0] * dV;
+        r[WDM_RED_VEXT] = ws[1] * dV;
+        if (nout > WDM_RED_PHIMAX && sg) {
+            WDM_LAUNCH((k_abs_max_sum), dim3(nb), dim3(256), p->stream, (const double*)phi, p->ncell,
+                       p->partials.as<double>());
+            hip_check(hipGetLastError(), "phi max");
+            double pm[2];
+            finalize(p, int(nb), pm);
+            r[WDM_RED_PHIMAX] = pm[1];
+        }
+    }
+    for (int i = 0; i < nout && i < WDM_RED_COUNT; ++i) out[i] = r[i];
+}
+
+template <class TD, class TS>
+void convert_dev(void* dst, const void* src, size_t n, hipStream_t s) {
+    WDM_LAUNCH((k_convert<TD, TS>), dim3(unsigned(std::min<size_t>((n + 255) / 256, 65535))), dim3(256), s,
+               static_cast<cx<TD>*>(dst), static_cast<const cx<TS>*>(src), n);
+    hip_check(hipGetLastError(), "convert");
+}
+
+template <class T>
+void set_psi_dev(wdm_plan* p, const void* src, int dtype, bool host) {
+    const size_t n = p->ncell;
+    cx<T>* d = p->psi.as<cx<T>>();
+    const bool same = (dtype == WDM_C128) == std::is_same<T, double>::value;
+    if (host) {
+        if (same) {
+            hip_check(hipMemcpyAsync(d, src, n * sizeof(cx<T>), hipMemcpyHostToDevice, p->stream), "set_psi");
+        } else {
+            std::vector<cx<T>> tmp(n);
+            if (dtype == WDM_C128) {
+                const double* s = static_cast<const double*>(src);
+                for (size_t i = 0; i < n; ++i) tmp[i] = mk(T(s[2 * i]), T(s[2 * i + 1]));
+            } else {
+                const float* s = static_cast<const float*>(src);
+                for (size_t i = 0; i < n; ++i) tmp[i] = mk(T(s[2 * i]), T(s[2 * i + 1]));
+            }
+            hip_check(hipMemcpyAsync(d, tmp.data(), n * sizeof(cx<T>), hipMemcpyHostToDevice, p->stream), "set_psi");
+            hip_check(hipStreamSynchronize(p->stream), "set_psi sync");
+        }
+    } else {
+        if (same) hip_check(hipMemcpyAsync(d, src, n * sizeof(cx<T>), hipMemcpyDeviceToDevice, p->stream), "set_psi");
+        else if (dtype == WDM_C128) convert_dev<T, double>(d, src, n, p->stream);
+        else convert_dev<T, float>(d, src, n, p->stream);
+    }
+    hip_check(hipStreamSynchronize(p->stream), "set_psi sync");
+    p->kspace = false;
+    p->phi_valid = false;
+    p->mass_valid = false;
+}
+
+template <class T>
+void get_psi_dev(wdm_plan* p, void* dst, int dtype, bool host) {
+    const size_t n = p->ncell;
+    const cx<T>* d = p->psi.as<cx<T>>();
+    const bool same = (dtype == WDM_C128) == std::is_same<T, double>::value;
+    if (host) {
+        if (same) {
+            hip_check(hipMemcpyAsync(dst, d, n * sizeof(cx<T>), hipMemcpyDeviceToHost, p->stream), "get_psi");
+            hip_check(hipStreamSynchronize(p->stream), "get_psi sync");
+        } else {
+            std::vector<cx<T>> tmp(n);
+            hip_check(hipMemcpyAsync(tmp.data(), d, n * sizeof(cx<T>), hipMemcpyDeviceToHost, p->stream), "get_psi");
+            hip_check(hipStreamSynchronize(p->stream), "get_psi sync");
+            if (dtype == WDM_C128) {
+                double* o = static_cast<double*>(dst);
+                for (size_t i = 0; i < n; ++i) { o[2 * i] = double(tmp[i].x); o[2 * i + 1] = double(tmp[i].y); }
+            } else {
+                float* o = static_cast<float*>(dst);
+                for (size_t i = 0; i < n; ++i) { o[2 * i] = float(tmp[i].x); o[2 * i + 1] = float(tmp[i].y); }
+            }
+        }
+    } else {
+        if (same) hip_check(hipMemcpyAsync(dst, d, n * sizeof(cx<T>), hipMemcpyDeviceToDevice, p->stream), "get_psi");
+        else if (dtype == WDM_C128) convert_dev<double, T>(dst, d, n, p->stream);
+        else convert_dev<float, T>(dst, d, n, p->stream);
+        hip_check(hipStreamSynchronize(p->stream), "get_psi sync");
+    }
+}
+
+template <class T>
+void poisson_api(wdm_plan* p, const double* rho, double* phi_out, double scale) {
+    typedef typename TwEntry<T>::type TwE;
+    if (rho == nullptr && phi_out == nullptr) {
+        // recompute Phi_hat of psi for the next kick
+        if (p->kspace) fail(WDM_ERR_STATE, "wdm_poisson: psi is in k-space");
+        if (scale != 0.0 && scale != p->four_pi_G) fail(WDM_ERR_ARG, "wdm_poisson: device result needs 4 pi G");
+        compute_phi<T>(p);
+        return;
+    }
+    const AxisOps<T>* ox = ops_of<T>(p, 0);
+    const int nlines = p->ny * p->nz;
+    if (scale == 0.0) scale = p->four_pi_G;
+    if (rho) {
+        double* src = p->scratch_real.get<double>(p->ncell);
+        hip_check(hipMemcpyAsync(src, rho, sizeof(double) * p->ncell, hipMemcpyHostToDevice, p->stream), "rho upload");
+        ox->x_r2c_real(src, p->rhohat.as<cx<T>>(), nlines, p->hpitch, p->ax[0].tw.as<TwE>(), p->stream);
+        hip_check(hipGetLastError(), "R2C pass");
+    } else {
+        if (p->kspace) fail(WDM_ERR_STATE, "wdm_poisson: psi is in k-space");
+        ox->x_dens(p->psi.as<cx<T>>(), p->rhohat.as<cx<T>>(), nlines, p->hpitch, p->ax[0].tw.as<TwE>(),
+                   p->partials.as<double>(), p->stream);
+        hip_check(hipGetLastError(), "density pass");
+    }
+    poisson_chain<T>(p, scale);
+    // rhohat now holds a user-scaled potential: not the kick's Phi unless it is
+    // the plan's own |psi|^2 with the plan's coefficient
+    p->phi_valid = (rho == nullptr && scale == p->four_pi_G);
+    if (phi_out) {
+        double* dst = p->scratch_real.get<double>(p->ncell);
+        ox->x_c2r_real(p->rhohat.as<cx<T>>(), dst, nlines, p->hpitch, p->ax[0].tw.as<TwE>(), p->stream);
+        hip_check(hipGetLastError(), "C2R pass");
+        hip_check(hipMemcpyAsync(phi_out, dst, sizeof(double) * p->ncell, hipMemcpyDeviceToHost, p->stream),
+                  "phi download");
+    }
+    hip_check(hipStreamSynchronize(p->stream), "poisson sync");
+}
+
+template <class T>
+void init_plan(wdm_plan* p) {
+    for (int a = 0; a < 3; ++a) {
+        AxisState& ax = p->ax[a];
+        ax.ops = axis_ops(ax.n, (T*)nullptr);
+        if (!ax.ops) fail(WDM_ERR_UNSUPPORTED, "axis length " + std::to_string(ax.n) + " is not supported");
+        build_twiddles<T>(ax);
+    }
+    p->psi.get<cx<T>>(p->ncell);
+    p->hpitch = ((p->nx / 2 + 1) + 7) / 8 * 8;
+    p->rhohat.get<cx<T>>(size_t(p->hpitch) * p->ny * p->nz);
+    // partial-sum slots: one pair per wave of the x passes (upper bound)
+    // exactly the waves an x pass launches; every launched wave writes its pair
+    const AxisOps<T>* ox = ops_of<T>(p, 0);
+    const size_t waves = (size_t(p->ny) * p->nz + ox->lpw - 1) / ox->lpw;
+    p->npartials = 2 * ((waves + kNW - 1) / kNW) * kNW;
+    p->partials.get<double>(std::max<size_t>(p->npartials, 2 * 1024));
+    p->redout.get<double>(2);
+}
+
+int fail_code(wdm_plan* p, int code, const std::string& msg) {
+    if (p) p->err = msg;
+    g_err = msg;
+    return code;
+}
+
+template <class F>
+int guarded(wdm_plan* p, F&& f) {
+    try {
+        f();
+        if (p) p->err.clear();
+        return WDM_OK;
+    } catch (const WdmError& e) {
+        return fail_code(p, e.code, e.msg);
+    } catch (const std::bad_alloc&) {
+        return fail_code(p, WDM_ERR_OOM, "host allocation failed");
+    } catch (const std::exception& e) {
+        return fail_code(p, WDM_ERR_HIP, e.what());
+    }
+}
+
+#define DISPATCH(p, FN, ...)                                   \
+    do {                                                       \
+        if ((p)->precision == WDM_F64) FN<double>(__VA_ARGS__); \
+        else FN<float>(__VA_ARGS__);                           \
+    } while (0)
+
+}  // namespace
+
+extern "C" {
+
+int wdm_version(void) { return WDM_ABI_VERSION; }
+
+const char* wdm_last_error(const wdm_plan* plan) {
+    if (plan) return plan->err.c_str();
+    return g_err.c_str();
+}
+
+int wdm_plan_create(wdm_plan** out, int nx, int ny, int nz, double xmax, double ymax, double zmax, int precision,
+                    unsigned flags) {
+    if (!out) return fail_code(nullptr, WDM_ERR_ARG, "wdm_plan_create: out is NULL");
+    *out = nullptr;
+    if (nx < 1 || ny < 1 || nz < 1) return fail_code(nullptr, WDM_ERR_ARG, "wdm_plan_create: bad grid size");
+    if (!(xmax > 0) || !(ymax > 0) || !(zmax > 0))
+        return fail_code(nullptr, WDM_ERR_ARG, "wdm_plan_create: box half-widths must be positive");
+    if (precision != WDM_F64 && precision != WDM_F32)
+        return fail_code(nullptr, WDM_ERR_ARG, "wdm_plan_create: precision must be WDM_F64 or WDM_F32");
+    if (double(nx) * ny * nz >= 2147483648.0)
+        return fail_code(nullptr, WDM_ERR_UNSUPPORTED, "wdm_plan_create: grid exceeds 2^31 cells");
+    wdm_plan* p = new (std::nothrow) wdm_plan();
+    if (!p) return fail_code(nullptr, WDM_ERR_OOM, "wdm_plan_create: out of host memory");
+    p->nx = nx;
+    p->ny = ny;
+    p->nz = nz;
+    p->ncell = size_t(nx) * ny * nz;
+    p->X[0] = xmax;
+    p->X[1] = ymax;
+    p->X[2] = zmax;
+    p->precision = precision;
+    p->flags = flags;
+    const int rc = guarded(nullptr, [&] {
+        hip_check(hipStreamCreate(&p->stream), "hipStreamCreate");
+        const int ns[3] = {nx, ny, nz};
+        for (int a = 0; a < 3; ++a) {
+            p->ax[a].n = ns[a];
+            std::vector<double> c(ns[a]);
+            for (int i = 0; i < ns[a]; ++i)
+                c[i] = (flags & WDM_GRID_CONSISTENT) ? -p->X[a] + i * (2.0 * p->X[a] / ns[a])
+                                                     : linrange_at(-p->X[a], p->X[a], ns[a], i);
+            if (flags & WDM_GRID_CONSISTENT) p->dx[a] = 2.0 * p->X[a] / ns[a];
+            else p->dx[a] = ns[a] > 1 ? c[1] - c[0] : 2.0 * p->X[a];
+            upload(p->ax[a].coord, c);
+        }
+        if (precision == WDM_F64) init_plan<double>(p);
+        else init_plan<float>(p);
+        build_k2(p);
+        hip_check(hipStreamSynchronize(p->stream), "plan init");
+    });
+    if (rc != WDM_OK) {
+        wdm_plan_destroy(p);
+        return rc;
+    }
+    *out = p;
+    return WDM_OK;
+}
+
+int wdm_plan_destroy(wdm_plan* p) {
+    if (!p) return WDM_OK;
+    if (p->stream) hipStreamSynchronize(p->stream);
+    for (auto& a : p->ax) {
+        a.tw.reset();
+        a.coord.reset();
+        a.drift.reset();
+        a.k2.reset();
+    }
+    for (DBuf* b : {&p->psi, &p->rhohat, &p->partials, &p->redout, &p->border, &p->vstatic, &p->scratch,
+                    &p->scratch_real})
+        b->reset();
+    if (p->stream) hipStreamDestroy(p->stream);
+    delete p;
+    return WDM_OK;
+}
+
+int wdm_set_coeffs(wdm_plan* p, double hbar_over_m, double four_pi_G) {
+    if (!p) return fail_code(nullptr, WDM_ERR_ARG, "plan is NULL");
+    if (!(hbar_over_m > 0) || !std::isfinite(four_pi_G))
+        return fail_code(p, WDM_ERR_ARG, "wdm_set_coeffs: hbar_over_m must be > 0 and four_pi_G finite");
+    p->hbar_over_m = hbar_over_m;
+    if (four_pi_G != p->four_pi_G) p->phi_valid = false;
+    p->four_pi_G = four_pi_G;
+    return WDM_OK;
+}
+
+int wdm_set_self_gravity(wdm_plan* p, int on) {
+    if (!p) return fail_code(nullptr, WDM_ERR_ARG, "plan is NULL");
+    p->self_gravity = on != 0;
+    p->phi_valid = false;
+    return WDM_OK;
+}
+
+int wdm_set_psi(wdm_plan* p, const void* host, int dtype) {
+    if (!p || !host) return fail_code(p, WDM_ERR_ARG, "wdm_set_psi: NULL argument");
+    if (dtype != WDM_C128 && dtype != WDM_C64) return fail_code(p, WDM_ERR_ARG, "wdm_set_psi: bad dtype");
+    return guarded(p, [&] { DISPATCH(p, set_psi_dev, p, host, dtype, true); });
+}
+
+int wdm_get_psi(wdm_plan* p, void* host, int dtype) {
+    if (!p || !host) return fail_code(p, WDM_ERR_ARG, "wdm_get_psi: NULL argument");
+    if (dtype != WDM_C128 && dtype != WDM_C64) return fail_code(p, WDM_ERR_ARG, "wdm_get_psi: bad dtype");
+    return guarded(p, [&] { DISPATCH(p, get_psi_dev, p, host, dtype, true); });
+}
+
+int wdm_set_psi_device(wdm_plan* p, const void* dev, int dtype) {
+    if (!p || !dev) return fail_code(p, WDM_ERR_ARG, "wdm_set_psi_device: NULL argument");
+    if (dtype != WDM_C128 && dtype != WDM_C64) return fail_code(p, WDM_ERR_ARG, "wdm_set_psi_device: bad dtype");
+    return guarded(p, [&] { DISPATCH(p, set_psi_dev, p, dev, dtype, false); });
+}
+
+int wdm_get_psi_device(wdm_plan* p, void* dev, int dtype) {
+    if (!p || !dev) return fail_code(p, WDM_ERR_ARG, "wdm_get_psi_device: NULL argument");
+    if (dtype != WDM_C128 && dtype != WDM_C64) return fail_code(p, WDM_ERR_ARG, "wdm_get_psi_device: bad dtype");
+    return guarded(p, [&] { DISPATCH(p, get_psi_dev, p, dev, dtype, false); });
+}
+
+int wdm_set_border(wdm_plan* p, const double* border, int rank) {
+    if (!p) return fail_code(nullptr, WDM_ERR_ARG, "plan is NULL");
+    if (!border) {
+        p->border_on = false;
+        return WDM_OK;
+    }
+    if (rank != 1) return fail_code(p, WDM_ERR_UNSUPPORTED, "wdm_set_border: only rank 1 (length nx) is supported");
+    return guarded(p, [&] {
+        hip_check(hipStreamSynchronize(p->stream), "border sync");
+        upload(p->border, std::vector<double>(border, border + p->nx));
+        p->border_on = true;
+    });
+}
+
+int wdm_set_static_potential(wdm_plan* p, const double* V) {
+    if (!p) return fail_code(nullptr, WDM_ERR_ARG, "plan is NULL");
+    if (!V) {
+        p->vflags &= ~V_STATIC;
+        return WDM_OK;
+    }
+    return guarded(p, [&] {
+        hip_check(hipStreamSynchronize(p->stream), "V sync");
+        double* d = p->vstatic.get<double>(p->ncell);
+        hip_check(hipMemcpy(d, V, sizeof(double) * p->ncell, hipMemcpyHostToDevice), "V upload");
+        p->vflags |= V_STATIC;
+    });
+}
+
+int wdm_set_potential_model(wdm_plan* p, int kind, const double* params, int nparams) {
+    if (!p) return fail_code(nullptr, WDM_ERR_ARG, "plan is NULL");
+    if (!params) return fail_code(p, WDM_ERR_ARG, "wdm_set_potential_model: params is NULL");
+    switch (kind) {
+        case WDM_V_HERNQUIST:
+            if (nparams != 5) return fail_code(p, WDM_ERR_ARG, "Hernquist takes 5 parameters (M, a, cx, cy, cz)");
+            std::copy(params, params + 5, p->hq);
+            p->vflags |= V_HERNQUIST;
+            return WDM_OK;
+        case WDM_V_QUADRUPOLE:
+            if (nparams != 13) return fail_code(p, WDM_ERR_ARG, "quadrupole takes 13 parameters");
+            std::copy(params, params + 13, p->quad);
+            p->vflags |= V_QUADRUPOLE;
+            return WDM_OK;
+        case WDM_V_KAPPA:
+            if (nparams != 1) return fail_code(p, WDM_ERR_ARG, "kappa takes 1 parameter");
+            p->kappa = params[0];
+            p->vflags |= V_KAPPA;
+            return WDM_OK;
+        default:
+            return fail_code(p, WDM_ERR_ARG, "unknown potential model " + std::to_string(kind));
+    }
+}
+
+int wdm_clear_potential_models(wdm_plan* p) {
+    if (!p) return fail_code(nullptr, WDM_ERR_ARG, "plan is NULL");
+    p->vflags &= V_STATIC;
+    return WDM_OK;
+}
+
+int wdm_poisson(wdm_plan* p, const double* rho, double* phi_out, double scale) {
+    if (!p) return fail_code(nullptr, WDM_ERR_ARG, "plan is NULL");
+    return guarded(p, [&] { DISPATCH(p, poisson_api, p, rho, phi_out, scale); });
+}
+
+int wdm_get_phi(wdm_plan* p, double* host_phi) {
+    if (!p || !host_phi) return fail_code(p, WDM_ERR_ARG, "wdm_get_phi: NULL argument");
+    if (p->kspace) return fail_code(p, WDM_ERR_STATE, "wdm_get_phi: psi is in k-space");
+    return guarded(p, [&] {
+        double* d = p->scratch_real.get<double>(p->ncell);
+        DISPATCH(p, phi_real, p, d);
+        hip_check(hipMemcpyAsync(host_phi, d, sizeof(double) * p->ncell, hipMemcpyDeviceToHost, p->stream), "phi");
+        hip_check(hipStreamSynchronize(p->stream), "phi sync");
+    });
+}
+
+int wdm_kick(wdm_plan* p, double dt_kick, double t) {
+    if (!p) return fail_code(nullptr, WDM_ERR_ARG, "plan is NULL");
+    if (!std::isfinite(dt_kick) || !std::isfinite(t)) return fail_code(p, WDM_ERR_ARG, "wdm_kick: non-finite argument");
+    return guarded(p, [&] {
+        DISPATCH(p, do_kick, p, dt_kick, t);
+        hip_check(hipStreamSynchronize(p->stream), "kick sync");
+    });
+}
+
+int wdm_drift(wdm_plan* p, double dt) {
+    if (!p) return fail_code(nullptr, WDM_ERR_ARG, "plan is NULL");
+    if (!std::isfinite(dt)) return fail_code(p, WDM_ERR_ARG, "wdm_drift: non-finite dt");
+    return guarded(p, [&] {
+        DISPATCH(p, do_drift, p, dt);
+        hip_check(hipStreamSynchronize(p->stream), "drift sync");
+    });
+}
+
+int wdm_step(wdm_plan* p, int nsteps, double dt, double dt_kick, double t0, int split) {
+    if (!p) return fail_code(nullptr, WDM_ERR_ARG, "plan is NULL");
+    if (nsteps < 0 || !std::isfinite(dt) || !std::isfinite(dt_kick) || !std::isfinite(t0))
+        return fail_code(p, WDM_ERR_ARG, "wdm_step: bad argument");
+    if (split != WDM_SPLIT_REFERENCE && split != WDM_SPLIT_STRANG)
+        return fail_code(p, WDM_ERR_ARG, "wdm_step: unknown split mode");
+    return guarded(p, [&] {
+        DISPATCH(p, do_step, p, nsteps, dt, dt_kick, t0, split);
+        hip_check(hipStreamSynchronize(p->stream), "step sync");
+        if (p->mass_valid) {
+            double ms[2];
+            finalize(p, int(p->npartials / 2), ms);
+            if (!std::isfinite(ms[0]) || !std::isfinite(ms[1]))
+                fail(WDM_ERR_NONFINITE, "wdm_step: psi became non-finite");
+        }
+    });
+}
+
+int wdm_reduce(wdm_plan* p, double* out, int nout) {
+    if (!p || !out || nout <= 0) return fail_code(p, WDM_ERR_ARG, "wdm_reduce: bad argument");
+    return guarded(p, [&] { DISPATCH(p, reduce_all, p, out, nout); });
+}
+
+int wdm_cfl(wdm_plan* p, double kappa, double* out3) {
+    if (!p || !out3) return fail_code(p, WDM_ERR_ARG, "wdm_cfl: bad argument");
+    return guarded(p, [&] {
+        double r[WDM_RED_COUNT];
+        DISPATCH(p, reduce_all, p, r, WDM_RED_COUNT);
+        const double mh = 1.0 / p->hbar_over_m;
+        out3[0] = 2.0 * p->dx[0] * p->dx[0] / M_PI / p->hbar_over_m;
+        out3[1] = r[WDM_RED_PHIMAX] > 0 ? 2.0 * M_PI / (mh * r[WDM_RED_PHIMAX]) : INFINITY;
+        out3[2] = (kappa != 0.0 && r[WDM_RED_PSIMAX] > 0) ? 2.0 * M_PI / std::fabs(kappa) / mh / r[WDM_RED_PSIMAX]
+                                                           : INFINITY;
+    });
+}
+
+int wdm_synchronize(wdm_plan* p) {
+    if (!p) return fail_code(nullptr, WDM_ERR_ARG, "plan is NULL");
+    return guarded(p, [&] { hip_check(hipStreamSynchronize(p->stream), "synchronize"); });
+}
+
+}  // extern "C"

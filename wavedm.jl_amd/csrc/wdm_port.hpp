@@ -1,0 +1,36 @@
+// Build-mode switch: the HIP build for gfx950 (the product) or the CPU fiber
+// emulation build used by the host-side bounds-checking tests.
+#pragma once
+
+#include <type_traits>
+
+#if defined(__HIPCC__) || defined(__HIP__)
+#include <hip/hip_runtime.h>
+#define WDM_GPU 1
+#define WDM_FI __device__ __forceinline__
+#define WDM_HDFI __host__ __device__ __forceinline__
+#define WDM_KERNEL(nthreads) __global__ __launch_bounds__(nthreads)
+// Orders LDS traffic between the lanes of one wavefront (program order inside a
+// wave is kept by the LDS unit; this stops the compiler reordering across it).
+#define WDM_WAVE_SYNC()                                   \
+    do {                                                  \
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront"); \
+        __builtin_amdgcn_wave_barrier();                  \
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront"); \
+    } while (0)
+#define WDM_SHARED __shared__
+#else
+#include "emu/emu_runtime.hpp"
+#define WDM_EMU 1
+#define WDM_FI inline
+#define WDM_HDFI inline
+#define WDM_KERNEL(nthreads)
+#define WDM_WAVE_SYNC() emu::barrier()
+#define WDM_SHARED static
+#endif
+
+#define WDM_UNROLL _Pragma("unroll")
+
+namespace wdm {
+constexpr int kWave = 64;  // CDNA wavefront width
+}

@@ -1,0 +1,184 @@
+"""Python handle on a ``wdm_plan`` (the device-resident SSFM state).
+
+Arrays cross the boundary in the reference's layout: numpy arrays of shape
+(Nx, Ny, Nz) in Fortran order (x fastest), complex128 or complex64.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _capi as A
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Plan:
+    """One grid, one precision, one HIP stream, all buffers device-resident.
+
+    ``lib`` selects the native library (default: the HIP build)."""
+
+    def __init__(self, shape, xmax, precision="f64", *, consistent_grid=False,
+                 hbar_over_m=1.0, four_pi_G=4.0 * np.pi, self_gravity=True, lib=None):
+        self.lib = lib if lib is not None else A.load()
+        nx, ny, nz = (int(v) for v in shape)
+        if np.isscalar(xmax):
+            xmax = (float(xmax),) * 3
+        self.shape = (nx, ny, nz)
+        self.xmax = tuple(float(v) for v in xmax)
+        self.precision = precision
+        self.consistent_grid = consistent_grid
+        prec = {"f64": A.WDM_F64, "f32": A.WDM_F32}[precision]
+        flags = A.WDM_GRID_CONSISTENT if consistent_grid else 0
+        h = C.c_void_p()
+        rc = self.lib.wdm_plan_create(C.byref(h), nx, ny, nz, *self.xmax, prec, flags)
+        A.check(self.lib, None, rc)
+        self._h = h
+        self.set_coeffs(hbar_over_m, four_pi_G)
+        self.set_self_gravity(self_gravity)
+
+    # -- lifetime -------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.wdm_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _chk(self, rc):
+        A.check(self.lib, self._h, rc)
+
+    # -- configuration --------------------------------------------------------
+    @property
+    def ncell(self):
+        return self.shape[0] * self.shape[1] * self.shape[2]
+
+    def set_coeffs(self, hbar_over_m, four_pi_G):
+        self.hbar_over_m, self.four_pi_G = float(hbar_over_m), float(four_pi_G)
+        self._chk(self.lib.wdm_set_coeffs(self._h, self.hbar_over_m, self.four_pi_G))
+
+    def set_self_gravity(self, on: bool):
+        self._chk(self.lib.wdm_set_self_gravity(self._h, 1 if on else 0))
+
+    def set_border(self, border):
+        if border is None:
+            self._chk(self.lib.wdm_set_border(self._h, None, 1))
+            return
+        b = np.ascontiguousarray(border, dtype=np.float64).ravel()
+        if b.size != self.shape[0]:
+            raise ValueError("border must be a length-Nx vector (broadcast along x)")
+        self._chk(self.lib.wdm_set_border(self._h, _dptr(b), 1))
+
+    def set_static_potential(self, V):
+        if V is None:
+            self._chk(self.lib.wdm_set_static_potential(self._h, None))
+            return
+        v = np.asarray(V, dtype=np.float64)
+        if v.shape != self.shape:
+            raise ValueError(f"V must have shape {self.shape}")
+        v = np.ascontiguousarray(v.ravel(order="F"))
+        self._chk(self.lib.wdm_set_static_potential(self._h, _dptr(v)))
+
+    def set_hernquist(self, M, a, center=(0.0, 0.0, 0.0)):
+        p = np.array([M, a, *center], dtype=np.float64)
+        self._chk(self.lib.wdm_set_potential_model(self._h, A.WDM_V_HERNQUIST, _dptr(p), 5))
+
+    def set_quadrupole(self, T0, axis, period, center=(0.0, 0.0, 0.0)):
+        """V = 1/2 (x-c)^T R(t) T0 R(t)^T (x-c); T0 symmetric 3x3 (or the 6
+        components xx, yy, zz, xy, xz, yz), R(t) rotation about ``axis`` by
+        2 pi t / period."""
+        T0 = np.asarray(T0, dtype=np.float64)
+        if T0.shape == (3, 3):
+            T0 = np.array([T0[0, 0], T0[1, 1], T0[2, 2], T0[0, 1], T0[0, 2], T0[1, 2]])
+        p = np.array([*T0, *axis, period, *center], dtype=np.float64)
+        self._chk(self.lib.wdm_set_potential_model(self._h, A.WDM_V_QUADRUPOLE, _dptr(p), 13))
+
+    def set_kappa(self, kappa):
+        p = np.array([kappa], dtype=np.float64)
+        self._chk(self.lib.wdm_set_potential_model(self._h, A.WDM_V_KAPPA, _dptr(p), 1))
+
+    def clear_potential_models(self):
+        self._chk(self.lib.wdm_clear_potential_models(self._h))
+
+    # -- state ----------------------------------------------------------------
+    def set_psi(self, psi):
+        psi = np.asarray(psi)
+        if psi.shape != self.shape:
+            raise ValueError(f"psi must have shape {self.shape}, got {psi.shape}")
+        if psi.dtype == np.complex64:
+            buf, dt = np.ascontiguousarray(psi.ravel(order="F")), A.WDM_C64
+        else:
+            buf, dt = np.ascontiguousarray(psi.astype(np.complex128).ravel(order="F")), A.WDM_C128
+        self._chk(self.lib.wdm_set_psi(self._h, buf.ctypes.data_as(C.c_void_p), dt))
+
+    def get_psi(self, dtype=np.complex128):
+        out = np.empty(self.ncell, dtype=dtype)
+        dt = A.WDM_C64 if np.dtype(dtype) == np.complex64 else A.WDM_C128
+        self._chk(self.lib.wdm_get_psi(self._h, out.ctypes.data_as(C.c_void_p), dt))
+        return out.reshape(self.shape, order="F")
+
+    def set_psi_device(self, ptr: int, complex64: bool):
+        self._chk(self.lib.wdm_set_psi_device(self._h, C.c_void_p(ptr),
+                                              A.WDM_C64 if complex64 else A.WDM_C128))
+
+    def get_psi_device(self, ptr: int, complex64: bool):
+        self._chk(self.lib.wdm_get_psi_device(self._h, C.c_void_p(ptr),
+                                              A.WDM_C64 if complex64 else A.WDM_C128))
+
+    # -- operators ------------------------------------------------------------
+    def poisson(self, rho=None, scale=0.0, want_phi=True):
+        """scale * lap^-1 (rho - mean) (rho=None: |psi|^2).  Returns Phi (FP64)."""
+        rbuf = None
+        if rho is not None:
+            rho = np.asarray(rho, dtype=np.float64)
+            if rho.shape != self.shape:
+                raise ValueError("rho shape mismatch")
+            rbuf = np.ascontiguousarray(rho.ravel(order="F"))
+        out = np.empty(self.ncell, dtype=np.float64) if want_phi else None
+        self._chk(self.lib.wdm_poisson(self._h, _dptr(rbuf) if rbuf is not None else None,
+                                       _dptr(out) if out is not None else None, float(scale)))
+        return None if out is None else out.reshape(self.shape, order="F")
+
+    def get_phi(self):
+        out = np.empty(self.ncell, dtype=np.float64)
+        self._chk(self.lib.wdm_get_phi(self._h, _dptr(out)))
+        return out.reshape(self.shape, order="F")
+
+    def kick(self, dt_kick, t=0.0):
+        self._chk(self.lib.wdm_kick(self._h, float(dt_kick), float(t)))
+
+    def drift(self, dt):
+        self._chk(self.lib.wdm_drift(self._h, float(dt)))
+
+    def step(self, nsteps, dt, dt_kick=None, t0=0.0, split="reference"):
+        if dt_kick is None:
+            dt_kick = 0.5 * dt
+        mode = {"reference": A.WDM_SPLIT_REFERENCE, "strang": A.WDM_SPLIT_STRANG}[split]
+        self._chk(self.lib.wdm_step(self._h, int(nsteps), float(dt), float(dt_kick), float(t0), mode))
+
+    def reduce(self, n=A.WDM_RED_COUNT):
+        out = np.zeros(A.WDM_RED_COUNT, dtype=np.float64)
+        self._chk(self.lib.wdm_reduce(self._h, _dptr(out), int(n)))
+        keys = ["mass", "kinetic", "w_self", "v_ext", "psi_max", "phi_max"]
+        return {k: float(out[i]) for i, k in enumerate(keys[:n])}
+
+    def cfl(self, kappa=0.0):
+        out = np.zeros(3, dtype=np.float64)
+        self._chk(self.lib.wdm_cfl(self._h, float(kappa), _dptr(out)))
+        return tuple(float(v) for v in out)
+
+    def synchronize(self):
+        self._chk(self.lib.wdm_synchronize(self._h))
