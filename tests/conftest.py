@@ -9,6 +9,12 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 
+def pytest_addoption(parser):
+    parser.addoption("--emu", action="store_true", default=False,
+                     help="run GPU-marked tests against the CPU emulation debug build "
+                          "(libwavedm_cpuemu.so, needs the ASan runtime preloaded)")
+
+
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs a real MI355X (HIP library calls)")
     config.addinivalue_line("markers", "slow: long-running CPU test")
@@ -32,3 +38,12 @@ def load_wavedm():
 @pytest.fixture(scope="session")
 def wdm():
     return load_wavedm()
+
+
+@pytest.fixture(scope="session")
+def nativelib(wdm, request):
+    """The native library the GPU tests drive: the HIP build, or the CPU
+    emulation debug build when pytest runs with --emu."""
+    if request.config.getoption("--emu"):
+        return wdm._capi.load_emu()
+    return wdm._capi.load()

@@ -1,0 +1,33 @@
+"""Host-side bounds check of the kernels: the GPU parity suite run against the
+CPU fiber-emulation build of the same kernel source, compiled with
+AddressSanitizer (every global and LDS access is checked).  Runs in a
+subprocess because the ASan runtime must be preloaded.  Debug build only: the
+package never loads it; the GPU run of the same suite is the parity claim."""
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+SUBSET = ("roundtrip or (poisson and (8x8x8 or 32x16x8 or 160x16x8 or 12x8x384)) or "
+          "(step_fp64 and (16x16x16 or 128x8x16 or 8x384x16)) or kick_and_drift or strang or "
+          "quadrupole or static_potential or nls or reductions or error_paths")
+
+
+def test_gpu_suite_on_emulated_kernels(wdm):
+    emu = wdm._capi.EMU_LIB
+    if not os.path.exists(emu):
+        pytest.fail(f"{emu} missing: run __graft_entry__.build()")
+    gxx = lambda f: subprocess.run(["g++", f"-print-file-name={f}"], capture_output=True,  # noqa: E731
+                                   text=True).stdout.strip()
+    env = dict(os.environ)
+    env["LD_PRELOAD"] = f"{gxx('libasan.so')} {gxx('libstdc++.so')}"
+    env["ASAN_OPTIONS"] = "detect_leaks=0"
+    r = subprocess.run([sys.executable, "-m", "pytest", os.path.join(ROOT, "tests", "test_gpu_parity.py"),
+                        "-m", "gpu", "--emu", "-q", "-x", "-p", "no:cacheprovider", "-k", SUBSET],
+                       cwd=ROOT, env=env, capture_output=True, text=True, timeout=900)
+    tail = "\n".join(l for l in (r.stdout + r.stderr).splitlines() if "WARNING: ASan" not in l)[-3000:]
+    assert r.returncode == 0, tail
+    assert " passed" in r.stdout, tail

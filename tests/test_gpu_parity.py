@@ -1,0 +1,264 @@
+"""GPU parity: the HIP path through the C ABI against the FP64 oracle.
+
+Bars: FP64 plan <= 1e-12 relative (max-norm for fields after one operator,
+L2 after steps); FP32 plan against the FP64 oracle within the FP32 bounds
+written in each test.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import ssfm_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(8, 8, 8), (16, 16, 16), (32, 16, 8), (64, 64, 64), (128, 8, 16), (160, 16, 8),
+          (256, 8, 8), (384, 8, 8), (8, 384, 16), (12, 8, 384)]
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def rel_max(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def blob(shape, X, seed=0, amp=30.0, consistent=False):
+    rng = np.random.default_rng(seed)
+    x, y, z, _, _ = O.setup_grid(*X, *shape, consistent=consistent)
+    r2 = x[:, None, None] ** 2 + y[None, :, None] ** 2 + z[None, None, :] ** 2
+    return np.asfortranarray(amp * np.exp(-r2 / 2) * np.exp(1j * rng.uniform(-0.3, 0.3, shape)))
+
+
+X3 = (4.0, 3.5, 3.0)
+
+
+def test_roundtrip(wdm, nativelib):
+    psi = blob((16, 8, 8), X3)
+    with wdm.Plan(psi.shape, X3, "f64", lib=nativelib) as p:
+        p.set_psi(psi)
+        assert np.array_equal(p.get_psi(), psi)
+        p.set_psi(psi.astype(np.complex64))
+        assert np.array_equal(p.get_psi(np.complex64), psi.astype(np.complex64))
+    with wdm.Plan(psi.shape, X3, "f32", lib=nativelib) as p:
+        p.set_psi(psi)
+        assert np.array_equal(p.get_psi(np.complex64), psi.astype(np.complex64))
+
+
+SHAPE_IDS = ["x".join(map(str, s)) for s in SHAPES]
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=SHAPE_IDS)
+def test_poisson_matches_oracle(wdm, nativelib, shape):
+    psi = blob(shape, X3, seed=1)
+    _, _, _, delta, _ = O.setup_grid(*X3, *shape)
+    ref = O.compute_gravitational_potential(psi, delta).real
+    for prec, tol in (("f64", 1e-12), ("f32", 2e-5)):
+        with wdm.Plan(shape, X3, prec, lib=nativelib) as p:
+            p.set_psi(psi)
+            phi = p.get_phi()
+            assert rel_max(phi, ref) < tol, (prec, rel_max(phi, ref))
+
+
+def test_parallel_poisson_mirror_reference_sanity(wdm, nativelib):
+    # test/test_parallel.jl:70-97 through the mirror: 32^3 Gaussian, Delta=1, N=(32,32,32)
+    x = O.linrange(-np.pi, np.pi, 32)
+    rho = np.exp(-(x[:, None, None] ** 2 + x[None, :, None] ** 2 + x[None, None, :] ** 2))
+    phi = wdm.kdk.parallel_poisson((1.0, 1.0, 1.0), (32, 32, 32), rho)
+    assert phi.shape == (32, 32, 32) and np.iscomplexobj(phi) and np.all(np.isfinite(phi))
+    assert 1e-6 < np.abs(phi).max() < 100.0
+    ref = O.parallel_poisson((1.0, 1.0, 1.0), (32, 32, 32), rho)
+    assert rel_max(phi, ref) < 1e-12
+
+
+def _hernquist_V(M, a, c, G=1.0):
+    def V(xxx, yyy, zzz, ps, t):
+        return -G * M / (np.sqrt((xxx - c[0]) ** 2 + (yyy - c[1]) ** 2 + (zzz - c[2]) ** 2) + a)
+    return V
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=SHAPE_IDS)
+def test_step_fp64_matches_oracle(wdm, nativelib, shape):
+    psi = blob(shape, X3, seed=2)
+    c = (0.3, -0.2, 0.1)
+    with wdm.Plan(shape, X3, "f64", lib=nativelib) as p:
+        p.set_psi(psi)
+        p.set_hernquist(50.0, 1.0, c)
+        p.step(4, 2e-3)
+        out = p.get_psi()
+    ref = O.run_ssfm(psi, X3, 4, 2e-3, V=_hernquist_V(50.0, 1.0, c))
+    assert rel_l2(out, ref) < 1e-12, rel_l2(out, ref)
+
+
+def test_kick_and_drift_api_match_oracle(wdm, nativelib):
+    shape, dt = (32, 16, 8), 2e-3
+    psi = blob(shape, X3, seed=3)
+    _, _, _, delta, _ = O.setup_grid(*X3, *shape)
+    K = wdm.kdk
+    with wdm.Plan(shape, X3, "f64", lib=nativelib) as p:
+        p.set_psi(psi)
+        phi_all, phi_wdm, spec = K.apply_kick_step(p, dt=dt / 2, t=0.0)
+        spec_h = spec.get_psi()
+        lp = K.setup_fft_operators(*X3, *shape, dt)
+        p2 = K.apply_drift_step(spec, lp, None)
+        out = p2.get_psi()
+    phi_ref = O.compute_gravitational_potential(psi, delta)
+    spec_ref = O.apply_kick_step(psi, phi_ref, None, dt / 2)
+    out_ref = O.apply_drift_step(spec_ref, O.setup_fft_operators(*X3, *shape, dt), None)
+    assert rel_max(phi_all, phi_ref.real) < 1e-12
+    assert rel_max(spec_h, spec_ref) < 1e-12
+    assert rel_max(out, out_ref) < 1e-12
+
+
+def test_c0_config_100_steps_fp64(wdm, nativelib):
+    """64^3, L=20 kpc, m22=1 soliton with a 0.3 rad random phase field,
+    dt=1e-3 Gyr (Float32-rounded as the reference does), 100 reference-split
+    steps: <= 1e-12 relative L2 against the oracle."""
+    U, IC = wdm.units, wdm.ic
+    shape, X = (64, 64, 64), (10.0, 10.0, 10.0)
+    psi0 = IC.soliton_random_phase(shape, X, seed=7)
+    hm, fpg = U.hbar_over_m(1.0), U.FOUR_PI_G
+    dt = float(np.float32(1e-3))
+    with wdm.Plan(shape, X, "f64", hbar_over_m=hm, four_pi_G=fpg, lib=nativelib) as p:
+        p.set_psi(psi0)
+        p.step(100, dt)
+        out = p.get_psi()
+    ref = O.run_ssfm(psi0, X, 100, dt, hbar_over_m=hm, four_pi_G=fpg)
+    err = rel_l2(out, ref)
+    print(f"C0 64^3 x 100 steps: rel L2 = {err:.3e}")
+    assert err < 1e-12
+
+
+def test_strang_split_matches_oracle(wdm, nativelib):
+    shape, dt = (32, 16, 16), 2e-3
+    psi = blob(shape, X3, seed=4)
+    with wdm.Plan(shape, X3, "f64", lib=nativelib) as p:
+        p.set_psi(psi)
+        p.step(5, dt, split="strang")
+        out = p.get_psi()
+    ref = O.run_ssfm(psi, X3, 5, dt, split=O.SPLIT_STRANG)
+    assert rel_l2(out, ref) < 1e-12, rel_l2(out, ref)
+
+
+def _rot(axis, ang):
+    n = np.asarray(axis, float) / np.linalg.norm(axis)
+    K = np.array([[0, -n[2], n[1]], [n[2], 0, -n[0]], [-n[1], n[0], 0]])
+    return np.eye(3) + math.sin(ang) * K + (1 - math.cos(ang)) * K @ K
+
+
+def test_rotating_quadrupole_and_kappa_match_oracle(wdm, nativelib):
+    shape, dt = (16, 16, 32), 2e-3
+    psi = blob(shape, X3, seed=5, amp=3.0)
+    T0 = np.array([[30.0, 4.0, -2.0], [4.0, -20.0, 1.0], [-2.0, 1.0, 10.0]])
+    axis, period, cen, kappa = (0.2, 0.5, 1.0), 0.05, (0.1, 0.0, -0.2), -0.3
+
+    def V(xxx, yyy, zzz, ps, t):
+        R = _rot(axis, 2 * math.pi * t / period)
+        T = R @ T0 @ R.T
+        d = [xxx - cen[0], yyy - cen[1], zzz - cen[2]]
+        v = 0.5 * sum(T[i, j] * d[i] * d[j] for i in range(3) for j in range(3))
+        return v + kappa * np.abs(ps) ** 2
+
+    with wdm.Plan(shape, X3, "f64", lib=nativelib) as p:
+        p.set_psi(psi)
+        p.set_quadrupole(T0, axis, period, cen)
+        p.set_kappa(kappa)
+        p.step(6, dt, t0=0.01)
+        out = p.get_psi()
+    ref = O.run_ssfm(psi, X3, 6, dt, V=V, t0=0.01)
+    assert rel_l2(out, ref) < 1e-12, rel_l2(out, ref)
+
+
+def test_static_potential_and_border(wdm, nativelib):
+    shape, dt = (32, 32, 32), 2e-3
+    X = (4.0, 4.0, 4.0)
+    psi = blob(shape, X, seed=6)
+    x, y, z, _, _ = O.setup_grid(*X, *shape)
+    xxx, yyy, zzz, r = O.setup_coordinates(x, y, z)
+    V = 5.0 * np.sin(xxx) * np.cos(0.5 * yyy) + 0.3 * zzz
+    border = O.setup_absorption_boundary(*X, x, y, z, 10.0, dt)
+    with wdm.Plan(shape, X, "f64", lib=nativelib) as p:
+        p.set_psi(psi)
+        p.set_static_potential(V)
+        p.set_border(border)
+        p.step(3, dt)
+        out = p.get_psi()
+    ref = O.run_ssfm(psi, X, 3, dt, V=V, border=border)
+    assert rel_l2(out, ref) < 1e-12, rel_l2(out, ref)
+
+
+def test_embedded_1d_nls_soliton_closed_form(wdm, nativelib):
+    # kappa |psi|^2 with kappa=-1 on an n x 1 x 1 line, no gravity:
+    # a sech(a x) exp(i a^2 t/2) is exact (cf. test/simulation_SPE1D.jl:12-18,219)
+    n, X, T, a = 256, 20.0, 0.5, 1.3
+    x, _, _, _, _ = O.setup_grid(X, 1.0, 1.0, n, 1, 1, consistent=True)
+    psi0 = (a / np.cosh(a * x)).astype(complex)[:, None, None]
+    with wdm.Plan((n, 1, 1), (X, 1.0, 1.0), "f64", consistent_grid=True, self_gravity=False, lib=nativelib) as p:
+        p.set_psi(psi0)
+        p.set_kappa(-1.0)
+        p.step(200, T / 200, split="strang")
+        out = p.get_psi()
+    exact = psi0 * np.exp(0.5j * a * a * T)
+    assert rel_l2(out, exact) < 1e-3
+    Vf = lambda xxx, yyy, zzz, ps, t: -np.abs(ps) ** 2  # noqa: E731
+    ref = O.run_ssfm(psi0, (X, 1.0, 1.0), 200, T / 200, V=Vf, self_gravity=False,
+                     consistent_grid=True, split=O.SPLIT_STRANG)
+    assert rel_l2(out, ref) < 1e-12
+
+
+def test_reductions_and_cfl_match_oracle(wdm, nativelib):
+    shape = (32, 32, 16)
+    psi = blob(shape, X3, seed=8)
+    _, _, _, delta, dV = O.setup_grid(*X3, *shape)
+    with wdm.Plan(shape, X3, "f64", lib=nativelib) as p:
+        p.set_psi(psi)
+        red = p.reduce()
+        cfl = p.cfl(kappa=0.5)
+    K, W, _ = O.energies(psi, X3)
+    assert math.isclose(red["mass"], O.mass(psi, dV), rel_tol=1e-13)
+    assert math.isclose(red["kinetic"], K, rel_tol=1e-12)
+    assert math.isclose(red["w_self"], W, rel_tol=1e-12)
+    phi = O.compute_gravitational_potential(psi, delta)
+    _, _, _, lims = O.compute_timestep(delta[0], phi, 0.5, psi, 1.0, 10, False, 0.9)
+    for a, b in zip(cfl, lims):
+        assert math.isclose(a, b, rel_tol=1e-12)
+
+
+def test_fp32_against_fp64_oracle(wdm, nativelib):
+    """FP32 plan vs FP64 oracle, 64^3, 50 steps of the C0 recipe."""
+    U, IC = wdm.units, wdm.ic
+    shape, X = (64, 64, 64), (10.0, 10.0, 10.0)
+    psi0 = IC.soliton_random_phase(shape, X, seed=7)
+    hm, fpg = U.hbar_over_m(1.0), U.FOUR_PI_G
+    dt = float(np.float32(1e-3))
+    with wdm.Plan(shape, X, "f32", hbar_over_m=hm, four_pi_G=fpg, lib=nativelib) as p:
+        p.set_psi(psi0)
+        m0 = p.reduce(1)["mass"]
+        p.step(50, dt)
+        out = p.get_psi()
+        m1 = p.reduce(1)["mass"]
+    ref = O.run_ssfm(psi0, X, 50, dt, hbar_over_m=hm, four_pi_G=fpg)
+    err = rel_l2(out, ref)
+    drift = abs(m1 - m0) / m0
+    print(f"FP32 64^3 x 50: rel L2 {err:.3e}, mass drift {drift:.3e}")
+    assert err < 2e-5
+    assert drift < 1e-8
+
+
+def test_error_paths(wdm, nativelib):
+    from wavedm_amd._capi import WaveDMError, WDM_ERR_STATE, WDM_ERR_UNSUPPORTED
+    with wdm.Plan((16, 16, 16), 2.0, "f64", lib=nativelib) as p:
+        p.set_psi(blob((16, 16, 16), (2.0, 2.0, 2.0)))
+        with pytest.raises(WaveDMError) as e:
+            p.drift(1e-3)
+        assert e.value.code == WDM_ERR_STATE
+        p.kick(1e-3)
+        with pytest.raises(WaveDMError) as e:
+            p.step(1, 1e-3)
+        assert e.value.code == WDM_ERR_STATE
+        p.drift(1e-3)
+    with pytest.raises(WaveDMError) as e:
+        wdm.Plan((17, 16, 16), 2.0, "f64", lib=nativelib)
+    assert e.value.code == WDM_ERR_UNSUPPORTED

@@ -92,7 +92,7 @@ def _bind(path: str) -> C.CDLL:
 
 def load(path: str | None = None) -> C.CDLL:
     """The HIP library; raises if it has not been built."""
-    path = path or os.environ.get("WAVEDM_HIP_LIB", HIP_LIB)
+    path = path or HIP_LIB
     if path not in _cache:
         if not os.path.exists(path):
             raise ImportError(

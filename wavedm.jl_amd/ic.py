@@ -1,0 +1,127 @@
+"""Seeded initial conditions for the benchmark configurations.
+
+psi0 = sqrt(rho) exp(i theta) as in the reference's setup
+(src/ic/generation.jl:185-187, src/physics/schrodinger.jl:37-46), with the
+configuration-defined profiles: a soliton rho_c / (1 + 0.091 (r/r_c)^2)^8
+(the standard fuzzy-dark-matter core profile), Gaussian random phase fields and
+soliton superpositions.  Randomness comes from SplitMix64 (Vigna's public
+algorithm) so any host language can regenerate identical draws.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+_M64 = (1 << 64) - 1
+
+
+class SplitMix64:
+    def __init__(self, seed: int):
+        self.state = seed & _M64
+
+    def next_u64(self) -> int:
+        self.state = (self.state + 0x9E3779B97F4A7C15) & _M64
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+        return z ^ (z >> 31)
+
+    def uniform(self) -> float:
+        """Uniform in (0, 1]."""
+        return ((self.next_u64() >> 11) + 1) * (1.0 / 9007199254740992.0)
+
+    def normal(self) -> float:
+        """Box-Muller (cosine branch)."""
+        u1, u2 = self.uniform(), self.uniform()
+        return math.sqrt(-2.0 * math.log(u1)) * math.cos(2.0 * math.pi * u2)
+
+
+def grid_coords(shape, xmax, consistent=False):
+    """Per-axis coordinates of the plan grid (LinRange(-X, X, N) by default)."""
+    from .kdk import setup_grid
+    x, y, z, _, _ = setup_grid(*xmax, *shape, consistent=consistent)
+    return x, y, z
+
+
+def soliton_density(r, rho_c, r_c):
+    return rho_c / (1.0 + 0.091 * (r / r_c) ** 2) ** 8
+
+
+def random_phase_field(shape, kmax_modes: float, rms: float, seed: int) -> np.ndarray:
+    """Real Gaussian random field on the grid's discrete Fourier modes with
+    |n| <= kmax_modes (integer lattice vectors), Hermitian, no k=0 term,
+    rescaled to the requested rms."""
+    rng = SplitMix64(seed)
+    nx, ny, nz = shape
+    K = int(math.floor(kmax_modes))
+    coef = np.zeros(shape, dtype=np.complex128)
+    for a in range(-K, K + 1):
+        for b in range(-K, K + 1):
+            for c in range(-K, K + 1):
+                if a * a + b * b + c * c > kmax_modes * kmax_modes:
+                    continue
+                # one draw per +/- pair: keep the lexicographically positive vector
+                if (a, b, c) <= (0, 0, 0) and (a, b, c) != (0, 0, 0):
+                    continue
+                if (a, b, c) == (0, 0, 0):
+                    continue
+                re, im = rng.normal(), rng.normal()
+                coef[a % nx, b % ny, c % nz] += re + 1j * im
+                coef[(-a) % nx, (-b) % ny, (-c) % nz] += re - 1j * im
+    theta = np.fft.ifftn(coef).real
+    s = math.sqrt(float(np.mean(theta ** 2)))
+    return np.asfortranarray(theta * (rms / s if s > 0 else 0.0))
+
+
+def soliton_random_phase(shape, xmax, rho_c=1e8, r_c=1.0, sigma=0.3, kcut=8, seed=2024,
+                         consistent=False) -> np.ndarray:
+    """Centred soliton times exp(i theta), theta a Gaussian random phase field
+    (rms sigma rad, |n| <= kcut modes)."""
+    x, y, z = grid_coords(shape, xmax, consistent)
+    r = np.sqrt(x[:, None, None] ** 2 + y[None, :, None] ** 2 + z[None, None, :] ** 2)
+    amp = np.sqrt(soliton_density(r, rho_c, r_c))
+    theta = random_phase_field(shape, kcut, sigma, seed)
+    return np.asfortranarray(amp * np.exp(1j * theta))
+
+
+def soliton_set(n, seed, r_pos, rc_mu=math.log(0.8), rc_sigma=0.3, rho_c0=1e8, v_sigma_kms=20.0):
+    """Seeded parameters of n solitons: positions uniform in a ball of radius
+    r_pos, log-normal core radii, rho_c = rho_c0 (r_c/kpc)^-4, uniform phases,
+    Gaussian velocity components (km/s -> kpc/Gyr).  Returns a (n, 9) array:
+    x, y, z, r_c, rho_c, phase, vx, vy, vz."""
+    rng = SplitMix64(seed)
+    out = np.zeros((n, 9))
+    for i in range(n):
+        while True:
+            p = [2.0 * rng.uniform() - 1.0 for _ in range(3)]
+            if p[0] ** 2 + p[1] ** 2 + p[2] ** 2 <= 1.0:
+                break
+        rc = math.exp(rc_mu + rc_sigma * rng.normal())
+        out[i, 0:3] = [r_pos * v for v in p]
+        out[i, 3] = rc
+        out[i, 4] = rho_c0 * rc ** -4
+        out[i, 5] = 2.0 * math.pi * rng.uniform()
+        out[i, 6:9] = [v_sigma_kms * 1.0227121650537077 * rng.normal() for _ in range(3)]
+    return out
+
+
+def soliton_superposition(shape, xmax, params, hbar_over_m, consistent=False) -> np.ndarray:
+    """sum_i sqrt(rho_i(|x - x_i|)) exp(i (phase_i + k_i . x)), with minimum-image
+    distances in the periodic box and each velocity kick k_i = v_i/(hbar/m)
+    rounded to the box's Fourier lattice so psi0 stays periodic."""
+    x, y, z = grid_coords(shape, xmax, consistent)
+    L = [2.0 * v for v in xmax]
+    psi = np.zeros(shape, dtype=np.complex128)
+    for row in params:
+        cx_, cy_, cz_, rc, rhoc, ph, vx, vy, vz = row
+        d = []
+        for coord, c0, Li in ((x, cx_, L[0]), (y, cy_, L[1]), (z, cz_, L[2])):
+            dd = coord - c0
+            dd = dd - Li * np.round(dd / Li)
+            d.append(dd)
+        r = np.sqrt(d[0][:, None, None] ** 2 + d[1][None, :, None] ** 2 + d[2][None, None, :] ** 2)
+        k = [2.0 * math.pi / Li * round(v / hbar_over_m * Li / (2.0 * math.pi)) for v, Li in zip((vx, vy, vz), L)]
+        phase = ph + k[0] * x[:, None, None] + k[1] * y[None, :, None] + k[2] * z[None, None, :]
+        psi += np.sqrt(soliton_density(r, rhoc, rc)) * np.exp(1j * phase)
+    return np.asfortranarray(psi)
