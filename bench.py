@@ -1,0 +1,223 @@
+"""Benchmark: cell-steps/s of the fused FP32 SSFM step at 384^3 on one MI355X.
+
+Workload (the c2 configuration): 384^3 grid, L = 20 kpc, 128 seeded solitons,
+self-gravity plus a rotating tidal quadrupole, autoset dt, 1000 KDK steps.
+A "step" is one reference-split iteration: kick (Poisson-consistent Phi of the
+current psi + V(x,t)) and drift, i.e. the 8 pass kernels.
+
+Printed JSON line: value = cell-steps/s of the FP32 run (inputs resident in
+HBM, timed region = exactly --steps steps between synchronisations), plus
+  roofline      dominant kernel, algorithmic bytes / average launch time vs 8 TB/s
+  cpu_baseline  the oracle's restatement of the reference CPU step (numpy/scipy
+                pocketfft, FP64, all host cores) on a bounded sample of the
+                same workload
+  fp64          the FP64 (plain form) run of the same steps, its speed and the
+                FP32-vs-FP64 psi error, mass and energy drifts.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import math
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.join(ROOT, "wavedm.jl_amd")
+HBM_PEAK_GBS = 8000.0
+
+
+def load_pkg():
+    if "wavedm_amd" in sys.modules:
+        return sys.modules["wavedm_amd"]
+    spec = importlib.util.spec_from_file_location(
+        "wavedm_amd", os.path.join(PKG_DIR, "__init__.py"), submodule_search_locations=[PKG_DIR])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["wavedm_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def pass_bytes(shape, elem_bytes):
+    """Algorithmic HBM bytes per launch of each pass: every pass reads and
+    writes its array once; the Poisson chain works on the (Nx/2+1) half
+    spectrum."""
+    nx, ny, nz = shape
+    n = nx * ny * nz
+    nh = (nx // 2 + 1) * ny * nz
+    psi_rw = 2 * n * elem_bytes
+    half_rw = 2 * nh * elem_bytes
+    return {
+        "x_kick_fwd": psi_rw + nh * elem_bytes,       # psi r/w + Phi_hat read
+        "y_fwd": psi_rw,
+        "z_drift": psi_rw,
+        "y_inv": psi_rw,
+        "x_inv_density": psi_rw + nh * elem_bytes,    # psi r/w + rho_hat write
+        "poisson_y_fwd": half_rw,
+        "poisson_z": half_rw,
+        "poisson_y_inv": half_rw,
+    }
+
+
+def energy(red):
+    return red["kinetic"] + red["w_self"] + red["v_ext"]
+
+
+def run_gpu(W, cfg, precision, steps, warmup):
+    C = W.configs
+    with W.Plan(cfg["shape"], cfg["xmax"], precision, hbar_over_m=cfg["hbar_over_m"],
+                four_pi_G=cfg["four_pi_G"]) as p:
+        C.install(p, cfg)
+        dt, lims = (cfg["dt"], None) if cfg["dt"] else C.autoset_dt(p, cfg["steps"])
+        if warmup:
+            p.step(warmup, dt)
+            C.install(p, cfg)
+        red0 = p.reduce()
+        p.synchronize()
+        t0 = time.perf_counter()
+        passes, ev_ms = p.step_timed(steps, dt)
+        p.synchronize()
+        wall = time.perf_counter() - t0
+        red1 = p.reduce()
+        psi = p.get_psi(np.complex128)
+    return dict(dt=dt, cfl=lims, wall=wall, event_ms=ev_ms, passes=passes, red0=red0, red1=red1, psi=psi)
+
+
+def cpu_baseline(W, cfg, dt, nsteps):
+    """The oracle's op-for-op restatement of the reference CPU step, timed on a
+    bounded number of steps of the same workload (FP64, scipy pocketfft with
+    one worker per host core)."""
+    sys.path.insert(0, ROOT)
+    from oracle import ssfm_oracle as O
+    with W.Plan(cfg["shape"], cfg["xmax"], "f64", hbar_over_m=cfg["hbar_over_m"],
+                four_pi_G=cfg["four_pi_G"]) as p:
+        W.configs.install(p, cfg)
+        psi0 = p.get_psi()
+    V = None
+    pot = cfg["potential"]
+    if pot is not None and pot[0] == "quadrupole":
+        kw = pot[1]
+        T0, axis, period, cen = np.asarray(kw["T0"]), np.asarray(kw["axis"]), kw["period"], kw["center"]
+
+        def V(xxx, yyy, zzz, ps, t):
+            n = axis / np.linalg.norm(axis)
+            K = np.array([[0, -n[2], n[1]], [n[2], 0, -n[0]], [-n[1], n[0], 0]])
+            a = 2 * math.pi * t / period
+            R = np.eye(3) + math.sin(a) * K + (1 - math.cos(a)) * K @ K
+            T = R @ T0 @ R.T
+            d = (xxx - cen[0], yyy - cen[1], zzz - cen[2])
+            return 0.5 * sum(T[i, j] * d[i] * d[j] for i in range(3) for j in range(3))
+    elif pot is not None and pot[0] == "hernquist":
+        kw = pot[1]
+        G = cfg["four_pi_G"] / (4 * math.pi)
+
+        def V(xxx, yyy, zzz, ps, t):
+            c = kw["center"]
+            return -G * kw["M"] / (np.sqrt((xxx - c[0]) ** 2 + (yyy - c[1]) ** 2 + (zzz - c[2]) ** 2) + kw["a"])
+    cores = os.cpu_count() or 1
+    be = O.Backend("scipy", workers=cores)
+    # one untimed step (planner / page-in), then the timed sample
+    O.run_ssfm(psi0, cfg["xmax"], 1, dt, hbar_over_m=cfg["hbar_over_m"], four_pi_G=cfg["four_pi_G"], V=V, be=be)
+    t0 = time.perf_counter()
+    O.run_ssfm(psi0, cfg["xmax"], nsteps, dt, hbar_over_m=cfg["hbar_over_m"], four_pi_G=cfg["four_pi_G"],
+               V=V, be=be)
+    sec = time.perf_counter() - t0
+    n = int(np.prod(cfg["shape"]))
+    model = ""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        model = next((l.split(":", 1)[1].strip() for l in out.splitlines() if l.startswith("Model name")), "")
+    except Exception:
+        pass
+    return {"value": n * nsteps / sec, "unit": "cell-steps/s", "cores": cores, "kind": "port",
+            "sample": f"{nsteps} steps of the 384^3 c2 workload (after 1 untimed), FP64, "
+                      f"oracle restatement of the reference CPU step with scipy pocketfft "
+                      f"({cores} workers) standing in for FFTW; {sec / nsteps:.2f} s/step; CPU: {model}",
+            "ms_per_step": 1e3 * sec / nsteps}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--config", default="c2")
+    ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--skip-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU slab decomposition is a design note, not part of this build", file=sys.stderr)
+        sys.exit(2)
+    W = load_pkg()
+    cfg = W.configs.build(args.config)
+    n = int(np.prod(cfg["shape"]))
+    print(f"[bench] {cfg['name']} {cfg['shape']} FP32 warmup={args.warmup} steps={args.steps}", file=sys.stderr,
+          flush=True)
+    r32 = run_gpu(W, cfg, "f32", args.steps, args.warmup)
+    ms32 = 1e3 * r32["wall"] / args.steps
+    value = n * args.steps / r32["wall"]
+    bytes32 = pass_bytes(cfg["shape"], 8)
+    step_bytes32 = sum(bytes32.values())
+    dom = max(r32["passes"], key=r32["passes"].get)
+    dom_avg_ms = r32["passes"][dom] / args.steps
+    achieved = bytes32[dom] / (dom_avg_ms * 1e-3) / 1e9
+    passes = {k: {"avg_us": 1e3 * v / args.steps, "bytes": bytes32[k],
+                  "gbs": bytes32[k] / (v / args.steps * 1e-3) / 1e9 if v > 0 else None}
+              for k, v in r32["passes"].items()}
+    out = {
+        "metric": "cell-steps/s + HBM GB/s at 384^3 (FP32 vs FP64 SSFM); adjoint/forward cost",
+        "value": value,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms32,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded soliton superposition, device-generated)",
+        "config": {"workload": f"{cfg['name']}: FP32 SSFM {cfg['shape'][0]}^3, L={2 * cfg['xmax'][0]:g} kpc, "
+                               f"128 solitons, self-gravity + rotating tidal quadrupole, autoset dt, "
+                               f"{args.steps} reference-split KDK steps",
+                   "grid": list(cfg["shape"]), "dt_gyr": r32["dt"], "cfl_limits": r32["cfl"]},
+        "hbm_gbs_step": step_bytes32 / (ms32 * 1e-3) / 1e9,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "step_frac": step_bytes32 / (ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "passes": passes,
+    }
+    m0, m1 = r32["red0"]["mass"], r32["red1"]["mass"]
+    out["fp32"] = {"mass_drift": abs(m1 - m0) / m0, "energy0": energy(r32["red0"]),
+                   "energy1": energy(r32["red1"])}
+    if not args.skip_fp64:
+        print("[bench] FP64 leg", file=sys.stderr, flush=True)
+        r64 = run_gpu(W, cfg, "f64", args.steps, min(args.warmup, 5))
+        ms64 = 1e3 * r64["wall"] / args.steps
+        err = float(np.linalg.norm(r32["psi"] - r64["psi"]) / np.linalg.norm(r64["psi"]))
+        e32, e64 = energy(r32["red1"]), energy(r64["red1"])
+        out["fp64"] = {"ms_per_step": ms64, "value": n * args.steps / r64["wall"],
+                       "speedup_fp32_vs_fp64": ms64 / ms32,
+                       "psi_rel_l2_fp32_vs_fp64": err,
+                       "mass_drift_fp64": abs(r64["red1"]["mass"] - r64["red0"]["mass"]) / r64["red0"]["mass"],
+                       "energy_rel_diff_fp32_vs_fp64": abs(e32 - e64) / abs(e64),
+                       "hbm_gbs_step": 2 * step_bytes32 / (ms64 * 1e-3) / 1e9}
+        del r64
+    del r32
+    if not args.skip_cpu:
+        print("[bench] CPU baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(W, cfg, out["config"]["dt_gyr"], args.cpu_steps)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

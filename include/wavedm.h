@@ -136,6 +136,30 @@ int wdm_cfl(wdm_plan* plan, double kappa, double* out3);
 
 int wdm_synchronize(wdm_plan* plan);
 
+/* wdm_step with HIP events around every kernel of the step on the plan's
+   stream: pass_ms[i] (i < min(npass, WDM_PASS_COUNT)) receives the summed
+   kernel time of pass i over the nsteps steps, in milliseconds; total_ms the
+   event time from the first to the last launch.  Reference split only. */
+#define WDM_PASS_XKICK 0    /* x: Phi C2R + kick + forward FFT_x          */
+#define WDM_PASS_YFWD 1     /* y: FFT_y                                   */
+#define WDM_PASS_ZDRIFT 2   /* z: FFT_z * drift * IFFT_z                  */
+#define WDM_PASS_YINV 3     /* y: IFFT_y                                  */
+#define WDM_PASS_XINV 4     /* x: IFFT_x + absorber + density R2C_x       */
+#define WDM_PASS_PYFWD 5    /* Poisson y forward (half spectrum)          */
+#define WDM_PASS_PZ 6       /* Poisson z: FFT_z * -4 pi G/k^2 * IFFT_z    */
+#define WDM_PASS_PYINV 7    /* Poisson y inverse                          */
+#define WDM_PASS_COUNT 8
+int wdm_step_timed(wdm_plan* plan, int nsteps, double dt, double dt_kick, double t0, double* pass_ms,
+                   int npass, double* total_ms);
+
+/* Initial condition on the device (FP64 evaluation, stored in the plan
+   precision): psi = sum_i sqrt(rho_c,i / (1 + 0.091 (r_i/r_c,i)^2)^8)
+   * exp(i (phase_i + k_i . x)), r_i the minimum-image distance to the
+   centre in the periodic box.  params: nsol rows of
+   {x0, y0, z0, r_c, rho_c, phase, kx, ky, kz}.  Replaces evaluating the IC
+   function on the host grid (src/physics/schrodinger.jl:37-46). */
+int wdm_ic_solitons(wdm_plan* plan, const double* params, int nsol);
+
 #ifdef __cplusplus
 }
 #endif

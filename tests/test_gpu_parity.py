@@ -14,7 +14,7 @@ from oracle import ssfm_oracle as O
 pytestmark = pytest.mark.gpu
 
 SHAPES = [(8, 8, 8), (16, 16, 16), (32, 16, 8), (64, 64, 64), (128, 8, 16), (160, 16, 8),
-          (256, 8, 8), (384, 8, 8), (8, 384, 16), (12, 8, 384)]
+          (256, 8, 8), (384, 8, 8), (8, 384, 16), (12, 8, 384), (24, 48, 96), (192, 8, 24)]
 
 
 def rel_l2(a, b):
@@ -262,3 +262,15 @@ def test_error_paths(wdm, nativelib):
     with pytest.raises(WaveDMError) as e:
         wdm.Plan((17, 16, 16), 2.0, "f64", lib=nativelib)
     assert e.value.code == WDM_ERR_UNSUPPORTED
+
+
+def test_device_soliton_ic_matches_host(wdm, nativelib):
+    IC = wdm.ic
+    shape, X = (32, 24, 16), (6.0, 5.0, 4.0)
+    hm = wdm.units.hbar_over_m(1.0)
+    params = IC.soliton_set(10, 123, 3.0)
+    ref = IC.soliton_superposition(shape, X, params, hm)
+    with wdm.Plan(shape, X, "f64", lib=nativelib) as p:
+        p.set_ic_solitons(IC.device_rows(params, X, hm))
+        out = p.get_psi()
+    assert rel_max(out, ref) < 1e-12, rel_max(out, ref)

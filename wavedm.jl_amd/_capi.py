@@ -70,7 +70,11 @@ SIGNATURES = {
     "wdm_reduce": (_I, [_P, _DP, _I]),
     "wdm_cfl": (_I, [_P, _D, _DP]),
     "wdm_synchronize": (_I, [_P]),
+    "wdm_step_timed": (_I, [_P, _I, _D, _D, _D, _DP, _I, _DP]),
+    "wdm_ic_solitons": (_I, [_P, _DP, _I]),
 }
+PASS_NAMES = ["x_kick_fwd", "y_fwd", "z_drift", "y_inv", "x_inv_density", "poisson_y_fwd", "poisson_z",
+              "poisson_y_inv"]
 
 _cache: dict[str, C.CDLL] = {}
 

@@ -131,10 +131,14 @@ struct Launch {
     X(8, 8, 1)               \
     X(12, 12, 1)             \
     X(16, 16, 1)             \
+    X(24, 24, 1)             \
     X(32, 8, 4)              \
+    X(48, 12, 4)             \
     X(64, 8, 8)              \
+    X(96, 24, 4)             \
     X(128, 16, 8)            \
     X(160, 40, 4)            \
+    X(192, 24, 8)            \
     X(256, 32, 8)            \
     X(384, 48, 8)
 

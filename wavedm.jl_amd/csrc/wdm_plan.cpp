@@ -89,6 +89,24 @@ struct AxisState {
     DBuf k2;                    // FP64 k^2 (FFT order; x axis: half spectrum)
 };
 
+// HIP events recorded after each pass launch (wdm_step_timed)
+struct PassTimer {
+    std::vector<hipEvent_t> ev;
+    std::vector<int> id;
+    size_t used = 0;
+    void reserve(size_t n) {
+        while (ev.size() < n) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) break;
+            ev.push_back(e);
+        }
+        id.resize(ev.size());
+    }
+    ~PassTimer() {
+        for (hipEvent_t e : ev) hipEventDestroy(e);
+    }
+};
+
 struct wdm_plan {
     int nx, ny, nz;
     size_t ncell;
@@ -116,10 +134,19 @@ struct wdm_plan {
     double quad[13] = {0};
     double kappa = 0.0;
     double last_kick_t = 0.0;
+    PassTimer* timer = nullptr;
     std::string err;
 };
 
 namespace {
+
+void mark(wdm_plan* p, int pass) {
+    if (!p->timer) return;
+    PassTimer& t = *p->timer;
+    if (t.used >= t.ev.size()) return;
+    hip_check(hipEventRecord(t.ev[t.used], p->stream), "hipEventRecord");
+    t.id[t.used++] = pass;
+}
 
 template <class T>
 const AxisOps<T>* ops_of(const wdm_plan* p, int a) {
@@ -210,9 +237,11 @@ void poisson_chain(wdm_plan* p, double scale) {
                 -scale / double(p->ncell)};
     oz->col(COL_FMI, KF_POISSON, rh, nh, p->ny, unsigned(hp * p->ny), unsigned(hp), p->ax[2].tw.as<TwE>(), kt,
             p->stream);
+    mark(p, WDM_PASS_PZ);
     if (p->ny > 1)
         oy->col(COL_INV, KF_PNONE, rh, nh, p->nz, unsigned(hp), unsigned(hp * p->ny), p->ax[1].tw.as<TwE>(), none,
                 p->stream);
+    mark(p, WDM_PASS_PYINV);
     launch_check<T>(p, "poisson chain");
 }
 
@@ -316,6 +345,7 @@ void x_kick(wdm_plan* p, bool fwd, double dt_kick, double t) {
     ox->x_kick(fwd, p->psi.as<cx<T>>(), p->rhohat.as<cx<T>>(), p->ny * p->nz, p->hpitch, p->ax[0].tw.as<TwE>(), ka,
                p->stream);
     hip_check(hipGetLastError(), "kick pass");
+    mark(p, WDM_PASS_XKICK);
     p->last_kick_t = t;
     p->phi_valid = false;
     p->mass_valid = false;
@@ -330,6 +360,7 @@ void y_pass(wdm_plan* p, int mode) {
     oy->col(mode, KF_NONE, p->psi.as<cx<T>>(), p->nx, p->nz, unsigned(p->nx), unsigned(p->nx * p->ny),
             p->ax[1].tw.as<TwE>(), none, p->stream);
     hip_check(hipGetLastError(), "y pass");
+    mark(p, mode == COL_FWD ? WDM_PASS_YFWD : WDM_PASS_YINV);
 }
 
 template <class T>
@@ -347,6 +378,7 @@ void z_pass(wdm_plan* p, int mode, double dt) {
     oz->col(mode, KF_NONE, p->psi.as<cx<T>>(), p->nx, p->ny, unsigned(p->nx * p->ny), unsigned(p->nx),
             p->ax[2].tw.as<TwE>(), kt, p->stream);
     hip_check(hipGetLastError(), "z pass");
+    mark(p, WDM_PASS_ZDRIFT);
 }
 
 // inverse x-pass + border + density (+ Poisson chain when self-gravity is on)
@@ -358,6 +390,7 @@ void x_inv_close(wdm_plan* p) {
                    p->hpitch, p->ax[0].tw.as<TwE>(), p->border_on ? p->border.as<double>() : nullptr,
                    p->partials.as<double>(), p->stream);
     hip_check(hipGetLastError(), "inverse x pass");
+    mark(p, WDM_PASS_XINV);
     p->kspace = false;
     p->mass_valid = p->self_gravity;
     if (p->self_gravity) {
@@ -616,6 +649,55 @@ void reduce_all(wdm_plan* p, double* out, int nout) {
         }
     }
     for (int i = 0; i < nout && i < WDM_RED_COUNT; ++i) out[i] = r[i];
+}
+
+template <class T>
+WDM_KERNEL(256)
+void k_ic_solitons(cx<T>* __restrict__ psi, int nx, int ny, int nz, const double* __restrict__ xs,
+                   const double* __restrict__ ys, const double* __restrict__ zs, const double* __restrict__ prm,
+                   int nsol, double Lx, double Ly, double Lz) {
+    const size_t n = size_t(nx) * ny * nz;
+    for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) {
+        const int x = int(i % nx), y = int((i / nx) % ny), z = int(i / (size_t(nx) * ny));
+        const double px = xs[x], py = ys[y], pz = zs[z];
+        double re = 0.0, im = 0.0;
+        for (int s = 0; s < nsol; ++s) {
+            const double* q = prm + 9 * s;
+            double dx = px - q[0], dy = py - q[1], dz = pz - q[2];
+            dx -= Lx * __builtin_rint(dx / Lx);
+            dy -= Ly * __builtin_rint(dy / Ly);
+            dz -= Lz * __builtin_rint(dz / Lz);
+            const double u = (dx * dx + dy * dy + dz * dz) / (q[3] * q[3]);
+            const double b = 1.0 + 0.091 * u;
+            const double b2 = b * b, b4 = b2 * b2;
+            const double amp = __builtin_sqrt(q[4]) / b4;  // sqrt(rho_c / b^8)
+            const double ph = q[5] + q[6] * px + q[7] * py + q[8] * pz;
+            int quad;
+            const double r = reduce_quadrant(ph, quad);
+            double sn, cs;
+            sincos_small(r, sn, cs);
+            // exp(i ph) = i^quad * (cs + i sn)
+            const cx<double> e = rot_quadrant(mk(cs, -sn), quad);  // (cs - i sn) * (-i)^quad = conj(exp(i ph))
+            re += amp * e.x;
+            im -= amp * e.y;
+        }
+        psi[i] = mk(T(re), T(im));
+    }
+}
+
+template <class T>
+void ic_solitons(wdm_plan* p, const double* params, int nsol) {
+    double* d = p->scratch_real.get<double>(std::max<size_t>(size_t(9) * nsol, 1));
+    hip_check(hipMemcpyAsync(d, params, sizeof(double) * 9 * nsol, hipMemcpyHostToDevice, p->stream), "ic params");
+    const unsigned nb = unsigned(std::min<size_t>((p->ncell + 255) / 256, 65536));
+    WDM_LAUNCH((k_ic_solitons<T>), dim3(nb), dim3(256), p->stream, p->psi.as<cx<T>>(), p->nx, p->ny, p->nz,
+               p->ax[0].coord.as<double>(), p->ax[1].coord.as<double>(), p->ax[2].coord.as<double>(),
+               (const double*)d, nsol, 2.0 * p->X[0], 2.0 * p->X[1], 2.0 * p->X[2]);
+    hip_check(hipGetLastError(), "ic kernel");
+    hip_check(hipStreamSynchronize(p->stream), "ic sync");
+    p->kspace = false;
+    p->phi_valid = false;
+    p->mass_valid = false;
 }
 
 template <class TD, class TS>
@@ -1013,6 +1095,47 @@ int wdm_cfl(wdm_plan* p, double kappa, double* out3) {
         out3[2] = (kappa != 0.0 && r[WDM_RED_PSIMAX] > 0) ? 2.0 * M_PI / std::fabs(kappa) / mh / r[WDM_RED_PSIMAX]
                                                            : INFINITY;
     });
+}
+
+int wdm_step_timed(wdm_plan* p, int nsteps, double dt, double dt_kick, double t0, double* pass_ms, int npass,
+                   double* total_ms) {
+    if (!p) return fail_code(nullptr, WDM_ERR_ARG, "plan is NULL");
+    if (nsteps < 0 || !std::isfinite(dt) || !std::isfinite(dt_kick) || !std::isfinite(t0))
+        return fail_code(p, WDM_ERR_ARG, "wdm_step_timed: bad argument");
+    if (p->kspace) return fail_code(p, WDM_ERR_STATE, "wdm_step_timed: psi is in k-space");
+    PassTimer timer;
+    const int rc = guarded(p, [&] {
+        // set-up work (Phi of the initial psi, drift tables) stays outside the timed window
+        DISPATCH(p, ensure_phi, p);
+        ensure_drift(p, dt);
+        hip_check(hipStreamSynchronize(p->stream), "timed sync");
+        timer.reserve(size_t(WDM_PASS_COUNT) * nsteps + 1);
+        p->timer = &timer;
+        mark(p, -1);
+        DISPATCH(p, do_step, p, nsteps, dt, dt_kick, t0, WDM_SPLIT_REFERENCE);
+        p->timer = nullptr;
+        hip_check(hipStreamSynchronize(p->stream), "timed sync");
+        double acc[WDM_PASS_COUNT] = {0};
+        for (size_t i = 1; i < timer.used; ++i) {
+            float ms = 0.f;
+            hip_check(hipEventElapsedTime(&ms, timer.ev[i - 1], timer.ev[i]), "hipEventElapsedTime");
+            if (timer.id[i] >= 0 && timer.id[i] < WDM_PASS_COUNT) acc[timer.id[i]] += ms;
+        }
+        for (int i = 0; i < npass && i < WDM_PASS_COUNT; ++i) pass_ms[i] = acc[i];
+        if (total_ms) {
+            float ms = 0.f;
+            if (timer.used > 1)
+                hip_check(hipEventElapsedTime(&ms, timer.ev[0], timer.ev[timer.used - 1]), "hipEventElapsedTime");
+            *total_ms = ms;
+        }
+    });
+    p->timer = nullptr;
+    return rc;
+}
+
+int wdm_ic_solitons(wdm_plan* p, const double* params, int nsol) {
+    if (!p || !params || nsol <= 0) return fail_code(p, WDM_ERR_ARG, "wdm_ic_solitons: bad argument");
+    return guarded(p, [&] { DISPATCH(p, ic_solitons, p, params, nsol); });
 }
 
 int wdm_synchronize(wdm_plan* p) {

@@ -106,6 +106,18 @@ def soliton_set(n, seed, r_pos, rc_mu=math.log(0.8), rc_sigma=0.3, rho_c0=1e8, v
     return out
 
 
+def device_rows(params, xmax, hbar_over_m):
+    """Rows for Plan.set_ic_solitons: velocity kicks rounded to the box's
+    Fourier lattice (the same rounding soliton_superposition applies)."""
+    L = [2.0 * v for v in xmax]
+    out = np.zeros((len(params), 9))
+    for i, row in enumerate(params):
+        cx_, cy_, cz_, rc, rhoc, ph, vx, vy, vz = row
+        k = [2.0 * math.pi / Li * round(v / hbar_over_m * Li / (2.0 * math.pi)) for v, Li in zip((vx, vy, vz), L)]
+        out[i] = [cx_, cy_, cz_, rc, rhoc, ph, *k]
+    return out
+
+
 def soliton_superposition(shape, xmax, params, hbar_over_m, consistent=False) -> np.ndarray:
     """sum_i sqrt(rho_i(|x - x_i|)) exp(i (phase_i + k_i . x)), with minimum-image
     distances in the periodic box and each velocity kick k_i = v_i/(hbar/m)

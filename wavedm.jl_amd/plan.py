@@ -169,6 +169,22 @@ class Plan:
         mode = {"reference": A.WDM_SPLIT_REFERENCE, "strang": A.WDM_SPLIT_STRANG}[split]
         self._chk(self.lib.wdm_step(self._h, int(nsteps), float(dt), float(dt_kick), float(t0), mode))
 
+    def step_timed(self, nsteps, dt, dt_kick=None, t0=0.0):
+        """Reference-split steps with HIP events around every pass; returns
+        (per-pass summed ms dict, total event ms)."""
+        if dt_kick is None:
+            dt_kick = 0.5 * dt
+        ms = np.zeros(len(A.PASS_NAMES), dtype=np.float64)
+        tot = np.zeros(1, dtype=np.float64)
+        self._chk(self.lib.wdm_step_timed(self._h, int(nsteps), float(dt), float(dt_kick), float(t0),
+                                          _dptr(ms), len(ms), _dptr(tot)))
+        return dict(zip(A.PASS_NAMES, ms.tolist())), float(tot[0])
+
+    def set_ic_solitons(self, params):
+        """Device-side soliton superposition (rows x0,y0,z0,r_c,rho_c,phase,kx,ky,kz)."""
+        prm = np.ascontiguousarray(np.asarray(params, dtype=np.float64).reshape(-1, 9))
+        self._chk(self.lib.wdm_ic_solitons(self._h, _dptr(prm), prm.shape[0]))
+
     def reduce(self, n=A.WDM_RED_COUNT):
         out = np.zeros(A.WDM_RED_COUNT, dtype=np.float64)
         self._chk(self.lib.wdm_reduce(self._h, _dptr(out), int(n)))
