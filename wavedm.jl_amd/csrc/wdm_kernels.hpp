@@ -16,6 +16,13 @@
 
 namespace wdm {
 
+// Arithmetic policy of the Poisson chain in FP32 plans.
+#ifdef WDM_POISSON_COMP
+typedef Comp PoissonPolicy;
+#else
+typedef Plain PoissonPolicy;
+#endif
+
 enum ColMode { COL_FWD = 0, COL_INV = 1, COL_FMI = 2, COL_MI = 3 };
 // KF_PNONE: no factor, Plain arithmetic (the y passes of the Poisson chain)
 enum KFactor { KF_NONE = 0, KF_DRIFT = 1, KF_POISSON = 2, KF_PNONE = 3 };
@@ -155,6 +162,15 @@ WDM_FI cx<double> kick_rot(cx<double> a, double theta) {
     return rot_quadrant(b, q);
 }
 WDM_FI cx<float> kick_rot(cx<float> a, double theta) {
+#ifdef WDM_KICK_F64
+    {
+        int q;
+        const double r = reduce_quadrant(theta, q);
+        double s, c;
+        sincos_small(r, s, c);
+        return rot_quadrant(mul_dyn<Comp>(a, c, -s), q);
+    }
+#endif
     int q;
     const float r = float(reduce_quadrant(theta, q));
     float s, c;
@@ -318,7 +334,7 @@ void k_x_kick(cx<T>* __restrict__ psi, const cx<T>* __restrict__ phihat, int nli
                 }
                 X[i][p] = ok ? zz : zero;
             }
-        inv2<F, +1, Plain>(X, v, m.j, xb, tw);
+        inv2<F, +1, PoissonPolicy>(X, v, m.j, xb, tw);
         WDM_UNROLL
         for (int kq = 0; kq < N1; ++kq) phi[kq] = v[kq].x;
     } else {
@@ -367,7 +383,7 @@ WDM_FI void density_r2c(const cx<T>* v, bool ok, const LaneMap& m, cx<T>* xb, cx
     }
     if (!ok) { msum = 0.0; mmax = 0.0; }
     cx<T> X[M][N2];
-    fwd2<F, -1, Plain>(r, X, m.j, xb, tw);
+    fwd2<F, -1, PoissonPolicy>(r, X, m.j, xb, tw);
     if (ok) {
         WDM_UNROLL
         for (int i = 0; i < M; ++i)
@@ -462,7 +478,7 @@ void k_x_r2c_real(const double* __restrict__ src, cx<T>* __restrict__ rhohat, in
     WDM_UNROLL
     for (int n1 = 0; n1 < N1; ++n1) v[n1] = mk(ok ? T(row[N2 * n1 + m.j]) : T(0), T(0));
     cx<T> X[M][N2];
-    fwd2<F, -1, Plain>(v, X, m.j, xb, tw);
+    fwd2<F, -1, PoissonPolicy>(v, X, m.j, xb, tw);
     if (ok) {
         cx<T>* h = rhohat + size_t(lc) * hpitch;
         WDM_UNROLL
@@ -505,7 +521,7 @@ void k_x_c2r_real(const cx<T>* __restrict__ hsrc, double* __restrict__ dst, int 
             }
             X[i][p] = ok ? zz : zero;
         }
-    inv2<F, +1, Plain>(X, v, m.j, xb, tw);
+    inv2<F, +1, PoissonPolicy>(X, v, m.j, xb, tw);
     if (ok) {
         double* row = dst + size_t(lc) * N;
         WDM_UNROLL

@@ -61,7 +61,8 @@ WDM_FI void fwd2(cx<T>* v, cx<T> (*X)[F::N2], int j, cx<T>* __restrict__ xb,
     Dft<N1, DIR, P, T>::run(v);
     if constexpr (N2 > 1) {
         WDM_UNROLL
-        for (int k1 = 1; k1 < N1; ++k1) v[k1] = mul_tab<DIR, P>(v[k1], tw[j * k1]);
+        for (int k1 = 1; k1 < N1; ++k1)
+            v[k1] = mul_tab<DIR, P>(v[k1], tw[j * k1], twiddle_may_hit_half(F::N, k1, 0, N2));
     }
     WDM_UNROLL
     for (int i = 0; i < M; ++i) {
@@ -85,7 +86,8 @@ WDM_FI void inv2(cx<T> (*X)[F::N2], cx<T>* v, int j, cx<T>* __restrict__ xb,
         if constexpr (N2 > 1) {
             const int q = i * N2 + j;
             WDM_UNROLL
-            for (int kp = 1; kp < N2; ++kp) X[i][kp] = mul_tab<DIR, P>(X[i][kp], tw[q * kp]);
+            for (int kp = 1; kp < N2; ++kp)
+                X[i][kp] = mul_tab<DIR, P>(X[i][kp], tw[q * kp], twiddle_may_hit_half(F::N, kp, i * N2 * kp, N2));
         }
     }
     WDM_UNROLL

@@ -80,6 +80,19 @@ constexpr CS cs2pi(long num, long den) {
             default: return {r, -r};
         }
     }
+    if ((12 * n) % den == 0) {
+        const double h3 = 0.86602540378443864676372317075294;  // sqrt(3)/2
+        switch ((12 * n) / den) {
+            case 1: return {h3, 0.5};
+            case 2: return {0.5, h3};
+            case 4: return {-0.5, h3};
+            case 5: return {-h3, 0.5};
+            case 7: return {-h3, -0.5};
+            case 8: return {-0.5, -h3};
+            case 10: return {0.5, -h3};
+            default: return {h3, -0.5};  // 11
+        }
+    }
     bool neg_s = false, neg_c = false, swap = false;
     long d = den;
     if (2 * n > d) { n = d - n; neg_s = true; }            // x -> 1 - x
@@ -108,6 +121,23 @@ struct TwTab {
 template <int R>
 inline constexpr TwTab<R> kTw{};
 
+// ---------------------------------------------------------------- hi/lo splits
+// v = hi + lo, hi = float(v), lo = float(v - hi).
+struct HL {
+    float h, l;
+};
+constexpr HL split_hl(double v) {
+    const float h = float(v);
+    return {h, float(v - double(h))};
+}
+// A power-of-two constant (the 0.5 of cos 60 deg) makes hi*x exact for any
+// FP32 x.  Small correction terms accumulated into an fma whose product is
+// exact always round away, so in a two-product sum the exact product must go
+// in the last fma and the corrections must ride on the inexact one.
+WDM_HDFI bool is_pow2f(float h) {
+    return h != 0.0f && (__builtin_bit_cast(unsigned, h) & 0x7FFFFFu) == 0u;
+}
+
 // ---------------------------------------------------------------- constant multiplies
 // x * k for a real constant k (given in FP64).
 template <class P, class T> WDM_FI T kmul(T x, double k);
@@ -115,10 +145,9 @@ template <> WDM_FI double kmul<Plain, double>(double x, double k) { return x * k
 template <> WDM_FI double kmul<Comp, double>(double x, double k) { return x * k; }
 template <> WDM_FI float kmul<Plain, float>(float x, double k) { return x * float(k); }
 template <> WDM_FI float kmul<Comp, float>(float x, double k) {
-    const float h = float(k);
-    const float l = float(k - double(h));
-    if (l == 0.0f) return x * h;
-    return fmaT(x, h, x * l);
+    const HL c = split_hl(k);
+    if (c.l == 0.0f) return x * c.h;
+    return fmaT(x, c.h, x * c.l);
 }
 template <class P, class T> WDM_FI cx<T> kmulc(cx<T> x, double k) {
     return mk(kmul<P, T>(x.x, k), kmul<P, T>(x.y, k));
@@ -130,16 +159,25 @@ template <class P, class T> WDM_FI cx<T> cmulk(cx<T> a, double c, double s) {
         const T cc = T(c), ss = T(s);
         return mk(fmaT(a.x, cc, -(a.y * ss)), fmaT(a.x, ss, a.y * cc));
     } else {
-        const float ch = float(c), cl = float(c - double(ch));
-        const float sh = float(s), sl = float(s - double(sh));
-        float t = a.x * cl;
-        t = fmaT(-a.y, sl, t);
-        t = fmaT(-a.y, sh, t);
-        t = fmaT(a.x, ch, t);
-        float u = a.x * sl;
-        u = fmaT(a.y, cl, u);
-        u = fmaT(a.y, ch, u);
-        u = fmaT(a.x, sh, u);
+        const HL cc = split_hl(c), ss = split_hl(s);
+        const float ch = cc.h, cl = cc.l, sh = ss.h, sl = ss.l;
+        // re = x c - y s ; im = x s + y c  (constants: the branches fold away)
+        float t = fmaT(-a.y, sl, a.x * cl);
+        if (is_pow2f(sh)) {
+            t = fmaT(a.x, ch, t);
+            t = fmaT(-a.y, sh, t);
+        } else {
+            t = fmaT(-a.y, sh, t);
+            t = fmaT(a.x, ch, t);
+        }
+        float u = fmaT(a.y, cl, a.x * sl);
+        if (is_pow2f(ch)) {
+            u = fmaT(a.x, sh, u);
+            u = fmaT(a.y, ch, u);
+        } else {
+            u = fmaT(a.y, ch, u);
+            u = fmaT(a.x, sh, u);
+        }
         return mk(t, u);
     }
 }
@@ -176,19 +214,45 @@ template <class T> struct TwEntry;
 template <> struct TwEntry<double> { typedef cx<double> type; };
 template <> struct TwEntry<float> { typedef TwF type; };
 
+// a * ((ch+cl) + i (sh+sl)) with the operation order chosen per value so that
+// a power-of-two hi goes in the last fma (see is_pow2f).  Lane-varying
+// constants need the selects; compile-time ones fold them away.
+WDM_FI cx<float> cmul_hl(cx<float> a, float ch, float cl, float sh, float sl) {
+    const bool snice = is_pow2f(sh), cnice = is_pow2f(ch);
+    float t = fmaT(-a.y, sl, a.x * cl);
+    t = fmaT(snice ? a.x : -a.y, snice ? ch : sh, t);
+    t = fmaT(snice ? -a.y : a.x, snice ? sh : ch, t);
+    float u = fmaT(a.y, cl, a.x * sl);
+    u = fmaT(cnice ? a.x : a.y, cnice ? sh : ch, u);
+    u = fmaT(cnice ? a.y : a.x, cnice ? ch : sh, u);
+    return mk(t, u);
+}
+
+// Can exp(-2 pi i (A j + B)/N), j in [0, NJ), fall on a multiple of 30 deg
+// that is not a multiple of 90 deg (a component of exactly +-0.5)?  Called
+// with compile-time arguments, so it folds to a constant.
+WDM_HDFI bool twiddle_may_hit_half(int N, int A, int B, int NJ) {
+    for (int j = 0; j < NJ; ++j) {
+        const int e = ((A * j + B) % N + N) % N;
+        if ((12 * e) % N == 0 && (4 * e) % N != 0) return true;
+    }
+    return false;
+}
+
 // x * (table entry)^DIR-conjugated: forward (DIR<0) uses exp(-i theta).
 template <int DIR, class P>
-WDM_FI cx<double> mul_tab(cx<double> a, const cx<double>& w) {
+WDM_FI cx<double> mul_tab(cx<double> a, const cx<double>& w, bool = false) {
     const double s = DIR > 0 ? w.y : -w.y;
     return mk(fmaT(a.x, w.x, -(a.y * s)), fmaT(a.x, s, a.y * w.x));
 }
 template <int DIR, class P>
-WDM_FI cx<float> mul_tab(cx<float> a, const TwF& w) {
+WDM_FI cx<float> mul_tab(cx<float> a, const TwF& w, bool robust = false) {
     const float sh = DIR > 0 ? w.sh : -w.sh;
     if constexpr (std::is_same<P, Plain>::value) {
         return mk(fmaT(a.x, w.ch, -(a.y * sh)), fmaT(a.x, sh, a.y * w.ch));
     } else {
         const float sl = DIR > 0 ? w.sl : -w.sl;
+        if (robust) return cmul_hl(a, w.ch, w.cl, sh, sl);
         float t = a.x * w.cl;
         t = fmaT(-a.y, sl, t);
         t = fmaT(-a.y, sh, t);
@@ -213,12 +277,8 @@ WDM_FI cx<float> mul_dyn(cx<float> a, double c, double s) {
         const float cc = float(c), ss = float(s);
         return mk(fmaT(a.x, cc, -(a.y * ss)), fmaT(a.x, ss, a.y * cc));
     } else {
-        TwF w;
-        w.ch = float(c);
-        w.cl = float(c - double(w.ch));
-        w.sh = float(s);
-        w.sl = float(s - double(w.sh));
-        return mul_tab<+1, Comp>(a, w);
+        const HL cc = split_hl(c), ss = split_hl(s);
+        return cmul_hl(a, cc.h, cc.l, ss.h, ss.l);
     }
 }
 

@@ -7,8 +7,20 @@
 #include "wdm_kernels.hpp"
 
 #ifdef WDM_GPU
-#define WDM_LAUNCH(KERNEL, GRID, BLOCK, STREAM, ...) \
-    hipLaunchKernelGGL(KERNEL, GRID, BLOCK, 0, STREAM, __VA_ARGS__)
+namespace wdm {
+// Optional start/stop events attached to the next pass launch (per-kernel
+// timing; null events launch normally).
+struct LaunchEvents {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+inline LaunchEvents& launch_events() {
+    static thread_local LaunchEvents ev;
+    return ev;
+}
+}  // namespace wdm
+#define WDM_LAUNCH(KERNEL, GRID, BLOCK, STREAM, ...)                                                      \
+    hipExtLaunchKernelGGL(KERNEL, GRID, BLOCK, 0, STREAM, wdm::launch_events().start, wdm::launch_events().stop, \
+                          0, __VA_ARGS__)
 #else
 #define WDM_LAUNCH(KERNEL, GRID, BLOCK, STREAM, ...) \
     emu::launch(GRID, BLOCK, [&]() { KERNEL(__VA_ARGS__); })
@@ -58,10 +70,11 @@ struct Launch {
         WDM_LAUNCH((k_col<T, PsiP, F, MODE, KF, kNW>), dim3(col_blocks<F>(ncols, nother)), dim3(kNW * 64), s, d,
                    ncols, nother, sl, so, tw, kt);
     }
+    typedef PoissonPolicy PoisP;
     template <int MODE, int KF>
     static void colp_t(cx<T>* d, int ncols, int nother, unsigned sl, unsigned so, const TwE* tw, KTabs kt,
                        hipStream_t s) {
-        WDM_LAUNCH((k_col<T, Plain, F, MODE, KF, kNW>), dim3(col_blocks<F>(ncols, nother)), dim3(kNW * 64), s, d,
+        WDM_LAUNCH((k_col<T, PoisP, F, MODE, KF, kNW>), dim3(col_blocks<F>(ncols, nother)), dim3(kNW * 64), s, d,
                    ncols, nother, sl, so, tw, kt);
     }
     // mode/kf combinations used by the plan; KF_POISSON passes run in Plain.

@@ -89,18 +89,18 @@ struct AxisState {
     DBuf k2;                    // FP64 k^2 (FFT order; x axis: half spectrum)
 };
 
-// HIP events recorded after each pass launch (wdm_step_timed)
+// Start/stop HIP events attached to each pass launch (wdm_step_timed).
 struct PassTimer {
-    std::vector<hipEvent_t> ev;
+    std::vector<hipEvent_t> ev;  // 2 per launch
     std::vector<int> id;
-    size_t used = 0;
-    void reserve(size_t n) {
-        while (ev.size() < n) {
+    size_t used = 0;             // launches recorded
+    void reserve(size_t launches) {
+        while (ev.size() < 2 * launches) {
             hipEvent_t e;
             if (hipEventCreate(&e) != hipSuccess) break;
             ev.push_back(e);
         }
-        id.resize(ev.size());
+        id.resize(ev.size() / 2);
     }
     ~PassTimer() {
         for (hipEvent_t e : ev) hipEventDestroy(e);
@@ -140,11 +140,26 @@ struct wdm_plan {
 
 namespace {
 
-void mark(wdm_plan* p, int pass) {
+// Arm the events for the next launch (call before it) / record which pass it
+// was (call after it).
+void arm(wdm_plan* p) {
+#ifdef WDM_GPU
     if (!p->timer) return;
     PassTimer& t = *p->timer;
-    if (t.used >= t.ev.size()) return;
-    hip_check(hipEventRecord(t.ev[t.used], p->stream), "hipEventRecord");
+    if (t.used >= t.id.size()) return;
+    launch_events().start = t.ev[2 * t.used];
+    launch_events().stop = t.ev[2 * t.used + 1];
+#else
+    (void)p;
+#endif
+}
+void mark(wdm_plan* p, int pass) {
+#ifdef WDM_GPU
+    launch_events() = LaunchEvents();
+#endif
+    if (!p->timer) return;
+    PassTimer& t = *p->timer;
+    if (t.used >= t.id.size()) return;
     t.id[t.used++] = pass;
 }
 
@@ -162,11 +177,12 @@ void build_twiddles(AxisState& a) {
         if constexpr (std::is_same<T, double>::value) {
             h[j] = mk(v.c, v.s);
         } else {
+            const HL c = split_hl(v.c), s = split_hl(v.s);
             TwF w;
-            w.ch = float(v.c);
-            w.cl = float(v.c - double(w.ch));
-            w.sh = float(v.s);
-            w.sl = float(v.s - double(w.sh));
+            w.ch = c.h;
+            w.cl = c.l;
+            w.sh = s.h;
+            w.sl = s.l;
             h[j] = w;
         }
     }
@@ -235,13 +251,16 @@ void poisson_chain(wdm_plan* p, double scale) {
     // Phi_k = -scale * rho_k / k^2, with the 1/N of the unnormalised inverse folded in
     KTabs kt = {p->ax[0].k2.as<double>(), p->ax[1].k2.as<double>(), p->ax[2].k2.as<double>(),
                 -scale / double(p->ncell)};
+    arm(p);
     oz->col(COL_FMI, KF_POISSON, rh, nh, p->ny, unsigned(hp * p->ny), unsigned(hp), p->ax[2].tw.as<TwE>(), kt,
             p->stream);
     mark(p, WDM_PASS_PZ);
-    if (p->ny > 1)
+    if (p->ny > 1) {
+        arm(p);
         oy->col(COL_INV, KF_PNONE, rh, nh, p->nz, unsigned(hp), unsigned(hp * p->ny), p->ax[1].tw.as<TwE>(), none,
                 p->stream);
-    mark(p, WDM_PASS_PYINV);
+        mark(p, WDM_PASS_PYINV);
+    }
     launch_check<T>(p, "poisson chain");
 }
 
@@ -342,6 +361,7 @@ void x_kick(wdm_plan* p, bool fwd, double dt_kick, double t) {
     typedef typename TwEntry<T>::type TwE;
     const AxisOps<T>* ox = ops_of<T>(p, 0);
     const KickArgs ka = make_kick<T>(p, dt_kick, t);
+    arm(p);
     ox->x_kick(fwd, p->psi.as<cx<T>>(), p->rhohat.as<cx<T>>(), p->ny * p->nz, p->hpitch, p->ax[0].tw.as<TwE>(), ka,
                p->stream);
     hip_check(hipGetLastError(), "kick pass");
@@ -357,6 +377,7 @@ void y_pass(wdm_plan* p, int mode) {
     if (p->ny == 1) return;
     const AxisOps<T>* oy = ops_of<T>(p, 1);
     KTabs none = {nullptr, nullptr, nullptr, 0.0};
+    arm(p);
     oy->col(mode, KF_NONE, p->psi.as<cx<T>>(), p->nx, p->nz, unsigned(p->nx), unsigned(p->nx * p->ny),
             p->ax[1].tw.as<TwE>(), none, p->stream);
     hip_check(hipGetLastError(), "y pass");
@@ -375,6 +396,7 @@ void z_pass(wdm_plan* p, int mode, double dt) {
     } else if (p->nz == 1) {
         return;
     }
+    arm(p);
     oz->col(mode, KF_NONE, p->psi.as<cx<T>>(), p->nx, p->ny, unsigned(p->nx * p->ny), unsigned(p->nx),
             p->ax[2].tw.as<TwE>(), kt, p->stream);
     hip_check(hipGetLastError(), "z pass");
@@ -386,6 +408,7 @@ template <class T>
 void x_inv_close(wdm_plan* p) {
     typedef typename TwEntry<T>::type TwE;
     const AxisOps<T>* ox = ops_of<T>(p, 0);
+    arm(p);
     ox->x_inv_dens(p->psi.as<cx<T>>(), p->self_gravity ? p->rhohat.as<cx<T>>() : nullptr, p->ny * p->nz,
                    p->hpitch, p->ax[0].tw.as<TwE>(), p->border_on ? p->border.as<double>() : nullptr,
                    p->partials.as<double>(), p->stream);
@@ -1109,23 +1132,23 @@ int wdm_step_timed(wdm_plan* p, int nsteps, double dt, double dt_kick, double t0
         DISPATCH(p, ensure_phi, p);
         ensure_drift(p, dt);
         hip_check(hipStreamSynchronize(p->stream), "timed sync");
-        timer.reserve(size_t(WDM_PASS_COUNT) * nsteps + 1);
+        timer.reserve(size_t(WDM_PASS_COUNT) * nsteps);
         p->timer = &timer;
-        mark(p, -1);
         DISPATCH(p, do_step, p, nsteps, dt, dt_kick, t0, WDM_SPLIT_REFERENCE);
         p->timer = nullptr;
         hip_check(hipStreamSynchronize(p->stream), "timed sync");
         double acc[WDM_PASS_COUNT] = {0};
-        for (size_t i = 1; i < timer.used; ++i) {
+        for (size_t i = 0; i < timer.used; ++i) {
             float ms = 0.f;
-            hip_check(hipEventElapsedTime(&ms, timer.ev[i - 1], timer.ev[i]), "hipEventElapsedTime");
+            hip_check(hipEventElapsedTime(&ms, timer.ev[2 * i], timer.ev[2 * i + 1]), "hipEventElapsedTime");
             if (timer.id[i] >= 0 && timer.id[i] < WDM_PASS_COUNT) acc[timer.id[i]] += ms;
         }
         for (int i = 0; i < npass && i < WDM_PASS_COUNT; ++i) pass_ms[i] = acc[i];
         if (total_ms) {
             float ms = 0.f;
-            if (timer.used > 1)
-                hip_check(hipEventElapsedTime(&ms, timer.ev[0], timer.ev[timer.used - 1]), "hipEventElapsedTime");
+            if (timer.used > 0)
+                hip_check(hipEventElapsedTime(&ms, timer.ev[0], timer.ev[2 * timer.used - 1]),
+                          "hipEventElapsedTime");
             *total_ms = ms;
         }
     });

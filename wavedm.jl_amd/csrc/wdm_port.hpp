@@ -6,10 +6,14 @@
 
 #if defined(__HIPCC__) || defined(__HIP__)
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #define WDM_GPU 1
 #define WDM_FI __device__ __forceinline__
 #define WDM_HDFI __host__ __device__ __forceinline__
-#define WDM_KERNEL(nthreads) __global__ __launch_bounds__(nthreads)
+#define WDM_KERNEL(nthreads) __global__ __launch_bounds__(nthreads, WDM_MIN_WAVES)
+#ifndef WDM_MIN_WAVES
+#define WDM_MIN_WAVES 1
+#endif
 // Orders LDS traffic between the lanes of one wavefront (program order inside a
 // wave is kept by the LDS unit; this stops the compiler reordering across it).
 #define WDM_WAVE_SYNC()                                   \
