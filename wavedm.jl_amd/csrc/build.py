@@ -81,9 +81,16 @@ def _jobs(kind, incremental):
     return jobs, objs
 
 
-def build(kind="hip", jobs=None, incremental=False, verbose=True):
+def build(kind="hip", jobs=None, incremental=False, verbose=True, defines=(), out=None):
+    """defines/out: extra -D flags and an alternate output .so (tuning variants)."""
+    global BUILD
     jobs = jobs or min(16, os.cpu_count() or 4)
+    saved = BUILD
+    if defines:
+        BUILD = os.path.join(saved, "var_" + "_".join(d.replace("-D", "").replace("=", "") for d in defines))
     cmds, objs = _jobs(kind, incremental)
+    BUILD = saved
+    cmds = [c[:1] + list(defines) + c[1:] for c in cmds]
     if verbose:
         print(f"[build:{kind}] compiling {len(cmds)} units with {jobs} jobs", flush=True)
     with cf.ThreadPoolExecutor(jobs) as ex:
@@ -91,10 +98,15 @@ def build(kind="hip", jobs=None, incremental=False, verbose=True):
             if verbose:
                 print(f"[build:{kind}] {os.path.basename(done)}", flush=True)
     os.makedirs(LIB, exist_ok=True)
-    if kind == "hip":
+    if out:
+        target = out
+        os.makedirs(os.path.dirname(out), exist_ok=True)
+        link = ([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}"] if kind == "hip"
+                else ["g++", "-shared", "-fPIC", "-fsanitize=address"]) + ["-o", target + ".tmp"] + objs
+    elif kind == "hip":
         target = os.path.join(LIB, "libwavedm_hip.so")
         link = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", target + ".tmp"] + objs
-    else:
+    if not out and kind == "emu":
         target = os.path.join(LIB, "libwavedm_cpuemu.so")
         link = ["g++", "-shared", "-fPIC", "-fsanitize=address", "-o", target + ".tmp"] + objs
     _run(link)

@@ -16,6 +16,17 @@
 
 namespace wdm {
 
+// Loads of lanes whose line is out of range: guarded (predicated, zero) or
+// unconditional from the clamped in-range address (never stored).
+#ifndef WDM_GUARDED_LOADS
+#define WDM_GUARDED_LOADS 1
+#endif
+#if WDM_GUARDED_LOADS
+#define WDM_LDG(OK, BASE, IDX) ((OK) ? ldg(BASE, IDX) : zero)
+#else
+#define WDM_LDG(OK, BASE, IDX) ldg(BASE, IDX)
+#endif
+
 // Arithmetic policy of the Poisson chain in FP32 plans.
 #ifdef WDM_POISSON_COMP
 typedef Comp PoissonPolicy;
@@ -248,7 +259,10 @@ void k_col(cx<T>* __restrict__ data, int ncols, int nother, unsigned sline, unsi
     const int c = tile * F::LPW + m.lw;
     const bool ok = (o < nother) && (c < ncols);
     const int oc = ok ? o : 0, cc = ok ? c : 0;
-    cx<T>* base = data + size_t(oc) * sother + cc;
+    // 32-bit element offsets from the uniform base pointer: one VGPR per
+    // address (scalar-base addressing) and cheap to recompute for the stores
+    const unsigned base = unsigned(oc) * sother + unsigned(cc) + unsigned(m.j) * sline;
+    const unsigned step = unsigned(N2) * sline;
     cx<T>* xb = &xbuf[m.wv][m.lw * F::XS];
     const cx<T> zero = mk(T(0), T(0));
     cx<T> v[N1];
@@ -257,31 +271,36 @@ void k_col(cx<T>* __restrict__ data, int ncols, int nother, unsigned sline, unsi
         WDM_UNROLL
         for (int n1 = 0; n1 < N1; ++n1) {
             const int k = N2 * n1 + m.j;
-            v[n1] = ok ? apply_kf<KF, P, T>(base[unsigned(k) * sline], kt, cc, oc, k) : zero;
+            v[n1] = apply_kf<KF, P, T>(WDM_LDG(ok, data, base + unsigned(n1) * step), kt, cc, oc, k);
         }
     } else {
         WDM_UNROLL
-        for (int n1 = 0; n1 < N1; ++n1) v[n1] = ok ? base[unsigned(N2 * n1 + m.j) * sline] : zero;
+        for (int n1 = 0; n1 < N1; ++n1) v[n1] = WDM_LDG(ok, data, base + unsigned(n1) * step);
     }
     constexpr int DIR = (MODE == COL_FWD || MODE == COL_FMI) ? -1 : +1;
     fwd2<F, DIR, P>(v, X, m.j, xb, tw);
     if constexpr (MODE == COL_FMI) {
         WDM_UNROLL
-        for (int i = 0; i < M; ++i)
+        for (int i = 0; i < M; ++i) {
+            WDM_SCHED_FENCE();
             WDM_UNROLL
             for (int k2 = 0; k2 < N2; ++k2)
                 X[i][k2] = apply_kf<KF, P, T>(X[i][k2], kt, cc, oc, (i * N2 + m.j) + N1 * k2);
+        }
         inv2<F, +1, P>(X, v, m.j, xb, tw);
         if (ok) {
+            const unsigned sbase = unsigned(oc) * sother + unsigned(cc) + unsigned(m.j) * sline;
             WDM_UNROLL
-            for (int kq = 0; kq < N1; ++kq) base[unsigned(N2 * kq + m.j) * sline] = v[kq];
+            for (int kq = 0; kq < N1; ++kq) stg(data, sbase + unsigned(kq) * step, v[kq]);
         }
     } else {
         if (ok) {
+            // X[i][k2] is element (i*N2 + j) + N1*k2 of the line
             WDM_UNROLL
             for (int i = 0; i < M; ++i)
                 WDM_UNROLL
-                for (int k2 = 0; k2 < N2; ++k2) base[unsigned((i * N2 + m.j) + N1 * k2) * sline] = X[i][k2];
+                for (int k2 = 0; k2 < N2; ++k2)
+                    stg(data, unsigned(oc) * sother + unsigned(cc) + unsigned((i * N2 + m.j) + N1 * k2) * sline, X[i][k2]);
         }
     }
 }
@@ -295,6 +314,8 @@ struct KickArgs {
     const double* xs;    // coordinates per axis (FP64)
     const double* ys;
     const double* zs;
+    double x0, x1;       // x(i) = (1 - t) x0 + t x1, t = i * xinv (the LinRange formula)
+    double xinv;
     VModel vm;
 };
 
@@ -312,14 +333,14 @@ void k_x_kick(cx<T>* __restrict__ psi, const cx<T>* __restrict__ phihat, int nli
     const bool ok = l < nlines;
     const int lc = ok ? l : 0;
     const int y = lc % ka.Ny, z = lc / ka.Ny;
-    cx<T>* row = psi + size_t(lc) * N;
+    const unsigned rbase = unsigned(lc) * unsigned(N);
     cx<T>* xb = &xbuf[m.wv][m.lw * F::XS];
     const cx<T> zero = mk(T(0), T(0));
     cx<T> X[M][N2];
     cx<T> v[N1];
     T phi[N1];
     if (ka.use_phi) {
-        const cx<T>* h = phihat + size_t(lc) * hpitch;
+        const unsigned hbase = unsigned(lc) * unsigned(hpitch);
         WDM_UNROLL
         for (int i = 0; i < M; ++i)
             WDM_UNROLL
@@ -327,12 +348,12 @@ void k_x_kick(cx<T>* __restrict__ psi, const cx<T>* __restrict__ phihat, int nli
                 const int k = i * N2 + m.j + N1 * p;
                 cx<T> zz;
                 if (2 * k <= N) {
-                    zz = h[k];
+                    zz = WDM_LDG(ok, phihat, hbase + unsigned(k));
                 } else {
-                    const cx<T> hh = h[N - k];
+                    const cx<T> hh = WDM_LDG(ok, phihat, hbase + unsigned(N - k));
                     zz = mk(hh.x, -hh.y);
                 }
-                X[i][p] = ok ? zz : zero;
+                X[i][p] = zz;
             }
         inv2<F, +1, PoissonPolicy>(X, v, m.j, xb, tw);
         WDM_UNROLL
@@ -342,12 +363,23 @@ void k_x_kick(cx<T>* __restrict__ psi, const cx<T>* __restrict__ phihat, int nli
         for (int kq = 0; kq < N1; ++kq) phi[kq] = T(0);
     }
     const double yc = ka.ys[y], zc = ka.zs[z];
+    // psi loads only after Phi is done (otherwise the 48 loads are hoisted
+    // above the C2R transform and both sets of registers are live at once)
+    WDM_SCHED_FENCE();
+    WDM_UNROLL
+    for (int n1 = 0; n1 < N1; ++n1) v[n1] = WDM_LDG(ok, psi, rbase + unsigned(N2 * n1 + m.j));
     WDM_UNROLL
     for (int n1 = 0; n1 < N1; ++n1) {
+        if ((n1 & 3) == 0) WDM_SCHED_FENCE();
         const int x = N2 * n1 + m.j;
-        const cx<T> a = ok ? row[x] : zero;
-        const double V = ka.vm.flags ? eval_V<T>(ka.vm, ka.xs[x], yc, zc, unsigned(lc) * N + x, a) : 0.0;
-        v[n1] = kick_rot(a, ka.mh_dt * (double(phi[n1]) + V));
+        double V = 0.0;
+        if (ka.vm.flags) {
+            const double tx = double(x) * ka.xinv;
+            V = eval_V<T>(ka.vm, (1.0 - tx) * ka.x0 + tx * ka.x1, yc, zc, unsigned(lc) * N + x, v[n1]);
+        }
+        T ph = phi[n1];
+        WDM_OPAQUE(ph);
+        v[n1] = kick_rot(v[n1], ka.mh_dt * (double(ph) + V));
     }
     if constexpr (FWD) {
         fwd2<F, -1, P>(v, X, m.j, xb, tw);
@@ -355,12 +387,12 @@ void k_x_kick(cx<T>* __restrict__ psi, const cx<T>* __restrict__ phihat, int nli
             WDM_UNROLL
             for (int i = 0; i < M; ++i)
                 WDM_UNROLL
-                for (int k2 = 0; k2 < N2; ++k2) row[(i * N2 + m.j) + N1 * k2] = X[i][k2];
+                for (int k2 = 0; k2 < N2; ++k2) stg(psi, rbase + unsigned((i * N2 + m.j) + N1 * k2), X[i][k2]);
         }
     } else {
         if (ok) {
             WDM_UNROLL
-            for (int n1 = 0; n1 < N1; ++n1) row[N2 * n1 + m.j] = v[n1];
+            for (int n1 = 0; n1 < N1; ++n1) stg(psi, rbase + unsigned(N2 * n1 + m.j), v[n1]);
         }
     }
 }
@@ -369,8 +401,8 @@ void k_x_kick(cx<T>* __restrict__ psi, const cx<T>* __restrict__ phihat, int nli
 // along x into the half-spectrum row h, plus FP64 sum/max partials per wave.
 template <class T, class F, int NW>
 WDM_FI void density_r2c(const cx<T>* v, bool ok, const LaneMap& m, cx<T>* xb, cx<T>* wbuf,
-                        const typename TwEntry<T>::type* __restrict__ tw, cx<T>* h,
-                        double* __restrict__ partials) {
+                        const typename TwEntry<T>::type* __restrict__ tw, cx<T>* __restrict__ hdata,
+                        unsigned hbase, double* __restrict__ partials) {
     constexpr int N1 = F::N1, N2 = F::N2, M = F::M, N = F::N;
     cx<T> r[N1];
     double msum = 0.0, mmax = 0.0;
@@ -390,7 +422,7 @@ WDM_FI void density_r2c(const cx<T>* v, bool ok, const LaneMap& m, cx<T>* xb, cx
             WDM_UNROLL
             for (int k2 = 0; k2 < N2; ++k2) {
                 const int k = (i * N2 + m.j) + N1 * k2;
-                if (2 * k <= N) h[k] = X[i][k2];
+                if (2 * k <= N) stg(hdata, hbase + unsigned(k), X[i][k2]);
             }
     }
     if (partials) {
@@ -417,7 +449,7 @@ void k_x_inv_dens(cx<T>* __restrict__ psi, cx<T>* __restrict__ rhohat, int nline
     const int l = m.w * F::LPW + m.lw;
     const bool ok = l < nlines;
     const int lc = ok ? l : 0;
-    cx<T>* row = psi + size_t(lc) * N;
+    const unsigned rbase = unsigned(lc) * unsigned(N);
     cx<T>* xb = &xbuf[m.wv][m.lw * F::XS];
     const cx<T> zero = mk(T(0), T(0));
     cx<T> X[M][N2];
@@ -425,7 +457,7 @@ void k_x_inv_dens(cx<T>* __restrict__ psi, cx<T>* __restrict__ rhohat, int nline
     WDM_UNROLL
     for (int i = 0; i < M; ++i)
         WDM_UNROLL
-        for (int p = 0; p < N2; ++p) X[i][p] = ok ? row[(i * N2 + m.j) + N1 * p] : zero;
+        for (int p = 0; p < N2; ++p) X[i][p] = WDM_LDG(ok, psi, rbase + unsigned((i * N2 + m.j) + N1 * p));
     inv2<F, +1, P>(X, v, m.j, xb, tw);
     if (border) {
         WDM_UNROLL
@@ -433,10 +465,10 @@ void k_x_inv_dens(cx<T>* __restrict__ psi, cx<T>* __restrict__ rhohat, int nline
     }
     if (ok) {
         WDM_UNROLL
-        for (int kq = 0; kq < N1; ++kq) row[N2 * kq + m.j] = v[kq];
+        for (int kq = 0; kq < N1; ++kq) stg(psi, rbase + unsigned(N2 * kq + m.j), v[kq]);
     }
     if (rhohat) {
-        density_r2c<T, F, NW>(v, ok, m, xb, &xbuf[m.wv][0], tw, rhohat + size_t(lc) * hpitch, partials);
+        density_r2c<T, F, NW>(v, ok, m, xb, &xbuf[m.wv][0], tw, rhohat, unsigned(lc) * unsigned(hpitch), partials);
     }
 }
 
@@ -452,13 +484,13 @@ void k_x_dens(const cx<T>* __restrict__ psi, cx<T>* __restrict__ rhohat, int nli
     const int l = m.w * F::LPW + m.lw;
     const bool ok = l < nlines;
     const int lc = ok ? l : 0;
-    const cx<T>* row = psi + size_t(lc) * N;
+    const unsigned rbase = unsigned(lc) * unsigned(N);
     cx<T>* xb = &xbuf[m.wv][m.lw * F::XS];
     const cx<T> zero = mk(T(0), T(0));
     cx<T> v[N1];
     WDM_UNROLL
-    for (int n1 = 0; n1 < N1; ++n1) v[n1] = ok ? row[N2 * n1 + m.j] : zero;
-    density_r2c<T, F, NW>(v, ok, m, xb, &xbuf[m.wv][0], tw, rhohat + size_t(lc) * hpitch, partials);
+    for (int n1 = 0; n1 < N1; ++n1) v[n1] = WDM_LDG(ok, psi, rbase + unsigned(N2 * n1 + m.j));
+    density_r2c<T, F, NW>(v, ok, m, xb, &xbuf[m.wv][0], tw, rhohat, unsigned(lc) * unsigned(hpitch), partials);
 }
 
 // R2C_x of a real FP64 source array (standalone Poisson solve of a host rho).
@@ -539,19 +571,19 @@ void k_x_fft(cx<T>* __restrict__ psi, int nlines, const typename TwEntry<T>::typ
     const int l = m.w * F::LPW + m.lw;
     const bool ok = l < nlines;
     const int lc = ok ? l : 0;
-    cx<T>* row = psi + size_t(lc) * N;
+    const unsigned rbase = unsigned(lc) * unsigned(N);
     cx<T>* xb = &xbuf[m.wv][m.lw * F::XS];
     const cx<T> zero = mk(T(0), T(0));
     cx<T> v[N1];
     cx<T> X[M][N2];
     WDM_UNROLL
-    for (int n1 = 0; n1 < N1; ++n1) v[n1] = ok ? row[N2 * n1 + m.j] : zero;
+    for (int n1 = 0; n1 < N1; ++n1) v[n1] = WDM_LDG(ok, psi, rbase + unsigned(N2 * n1 + m.j));
     fwd2<F, DIR, P>(v, X, m.j, xb, tw);
     if (ok) {
         WDM_UNROLL
         for (int i = 0; i < M; ++i)
             WDM_UNROLL
-            for (int k2 = 0; k2 < N2; ++k2) row[(i * N2 + m.j) + N1 * k2] = X[i][k2];
+            for (int k2 = 0; k2 < N2; ++k2) stg(psi, rbase + unsigned((i * N2 + m.j) + N1 * k2), X[i][k2]);
     }
 }
 

@@ -59,13 +59,18 @@ WDM_FI void fwd2(cx<T>* v, cx<T> (*X)[F::N2], int j, cx<T>* __restrict__ xb,
                  const typename TwEntry<T>::type* __restrict__ tw) {
     constexpr int N1 = F::N1, N2 = F::N2, M = F::M;
     Dft<N1, DIR, P, T>::run(v);
-    if constexpr (N2 > 1) {
-        WDM_UNROLL
-        for (int k1 = 1; k1 < N1; ++k1)
-            v[k1] = mul_tab<DIR, P>(v[k1], tw[j * k1], twiddle_may_hit_half(F::N, k1, 0, N2));
-    }
     WDM_UNROLL
     for (int i = 0; i < M; ++i) {
+        WDM_SCHED_FENCE();
+        if constexpr (N2 > 1) {
+            // twiddles of this chunk only (k1 = i*N2 + b)
+            WDM_UNROLL
+            for (int b = 0; b < N2; ++b) {
+                const int k1 = i * N2 + b;
+                if (k1 > 0)
+                    v[k1] = mul_tab<DIR, P>(v[k1], tw[j * k1], twiddle_may_hit_half(F::N, k1, 0, N2));
+            }
+        }
         cx<T> in[N2];
         xpose<N2>(xb, j, v + i * N2, in);
         Dft<N2, DIR, P, T>::run(in);
@@ -82,6 +87,7 @@ WDM_FI void inv2(cx<T> (*X)[F::N2], cx<T>* v, int j, cx<T>* __restrict__ xb,
     constexpr int N1 = F::N1, N2 = F::N2, M = F::M;
     WDM_UNROLL
     for (int i = 0; i < M; ++i) {
+        WDM_SCHED_FENCE();
         Dft<N2, DIR, P, T>::run(X[i]);
         if constexpr (N2 > 1) {
             const int q = i * N2 + j;

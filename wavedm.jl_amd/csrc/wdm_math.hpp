@@ -32,6 +32,18 @@ template <int S, class T> WDM_HDFI cx<T> mul_i(cx<T> a) {
 }
 template <class T> WDM_HDFI T norm2(cx<T> a) { return a.x * a.x + a.y * a.y; }
 
+// Element access at a 32-bit index from a wave-uniform base pointer: the byte
+// offset is formed in 32 bits so the backend can use scalar-base addressing
+// (one VGPR per address instead of a 64-bit pair).  Arrays stay below 4 GiB.
+template <class T>
+WDM_HDFI T ldg(const T* base, unsigned idx) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + idx * unsigned(sizeof(T)));
+}
+template <class T>
+WDM_HDFI void stg(T* base, unsigned idx, const T& v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + idx * unsigned(sizeof(T))) = v;
+}
+
 WDM_HDFI float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 WDM_HDFI double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
 

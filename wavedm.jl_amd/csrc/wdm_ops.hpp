@@ -28,7 +28,10 @@ inline LaunchEvents& launch_events() {
 
 namespace wdm {
 
-constexpr int kNW = 4;  // waves per workgroup for the pass kernels
+#ifndef WDM_NW
+#define WDM_NW 4
+#endif
+constexpr int kNW = WDM_NW;  // waves per workgroup for the pass kernels
 
 template <class T>
 struct AxisOps {

@@ -245,9 +245,12 @@ void poisson_chain(wdm_plan* p, double scale) {
     cx<T>* rh = p->rhohat.as<cx<T>>();
     const int nh = p->nx / 2 + 1, hp = p->hpitch;
     KTabs none = {nullptr, nullptr, nullptr, 0.0};
-    if (p->ny > 1)
+    if (p->ny > 1) {
+        arm(p);
         oy->col(COL_FWD, KF_PNONE, rh, nh, p->nz, unsigned(hp), unsigned(hp * p->ny), p->ax[1].tw.as<TwE>(), none,
                 p->stream);
+        mark(p, WDM_PASS_PYFWD);
+    }
     // Phi_k = -scale * rho_k / k^2, with the 1/N of the unnormalised inverse folded in
     KTabs kt = {p->ax[0].k2.as<double>(), p->ax[1].k2.as<double>(), p->ax[2].k2.as<double>(),
                 -scale / double(p->ncell)};
@@ -346,6 +349,16 @@ KickArgs make_kick(const wdm_plan* p, double dt_kick, double t) {
     ka.xs = p->ax[0].coord.as<double>();
     ka.ys = p->ax[1].coord.as<double>();
     ka.zs = p->ax[2].coord.as<double>();
+    // x coordinate in-kernel: LinRange(-X, X, nx) (or the consistent grid)
+    if (p->flags & WDM_GRID_CONSISTENT) {
+        ka.x0 = -p->X[0];
+        ka.x1 = p->X[0];
+        ka.xinv = 1.0 / double(p->nx);
+    } else {
+        ka.x0 = -p->X[0];
+        ka.x1 = p->X[0];
+        ka.xinv = p->nx > 1 ? 1.0 / double(p->nx - 1) : 0.0;
+    }
     ka.vm = make_vmodel(p, t);
     return ka;
 }

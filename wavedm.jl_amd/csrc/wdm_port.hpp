@@ -23,6 +23,13 @@
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront"); \
     } while (0)
 #define WDM_SHARED __shared__
+// Keeps the scheduler from hoisting the next chunk's table loads above this
+// point (bounds register pressure in the unrolled line FFTs).
+#define WDM_SCHED_FENCE() __builtin_amdgcn_sched_barrier(0)
+// Makes a register value opaque to the optimiser at this point, so work that
+// depends on it cannot be hoisted above (used to keep FP64 conversions at
+// their point of use).
+#define WDM_OPAQUE(x) asm volatile("" : "+v"(x))
 #else
 #include "emu/emu_runtime.hpp"
 #define WDM_EMU 1
@@ -31,6 +38,8 @@
 #define WDM_KERNEL(nthreads)
 #define WDM_WAVE_SYNC() emu::barrier()
 #define WDM_SHARED static
+#define WDM_SCHED_FENCE() ((void)0)
+#define WDM_OPAQUE(x) ((void)0)
 #endif
 
 #define WDM_UNROLL _Pragma("unroll")
