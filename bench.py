@@ -141,6 +141,27 @@ def cpu_baseline(W, cfg, dt, nsteps):
             "ms_per_step": 1e3 * sec / nsteps}
 
 
+def copy_calibration(mib, reps):
+    """A plain device-to-device copy of known size: the reference point for
+    the HBM byte counters (FETCH_SIZE / WRITE_SIZE) and the achievable copy
+    bandwidth.  Not part of the benchmark line."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    t1.record()
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1) / reps
+    print(json.dumps({"copy_bytes_each_way": 4 * n, "reps": reps, "avg_ms": ms,
+                      "gbs_read_plus_write": 2 * 4 * n / (ms * 1e-3) / 1e9}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -150,11 +171,16 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU slab decomposition is a design note, not part of this build", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration, max(args.steps, 1))
+        return
     W = load_pkg()
     cfg = W.configs.build(args.config)
     n = int(np.prod(cfg["shape"]))

@@ -99,6 +99,40 @@ WDM_FI double eval_V(const VModel& vm, double x, double y, double z, unsigned id
     return V;
 }
 
+// V along one x line (fixed y, z): the quadrupole is a quadratic in
+// dx = x - c_x with per-line coefficients, the Hernquist term needs only dx^2
+// on top of the line's transverse radius.  Same FP64 formulas as eval_V up to
+// the association of the quadrupole sum.
+struct LineV {
+    double qa, qb, qc;  // quadrupole: (qa*dx + qb)*dx + qc
+    double hr2;         // Hernquist: (y - c_y)^2 + (z - c_z)^2
+};
+WDM_FI LineV line_V(const VModel& vm, double y, double z) {
+    LineV l;
+    const double dy = y - vm.q_c[1], dz = z - vm.q_c[2];
+    l.qa = 0.5 * vm.qT[0];
+    l.qb = vm.qT[3] * dy + vm.qT[4] * dz;
+    l.qc = 0.5 * (vm.qT[1] * dy * dy + vm.qT[2] * dz * dz) + vm.qT[5] * dy * dz;
+    const double hy = y - vm.hq_c[1], hz = z - vm.hq_c[2];
+    l.hr2 = hy * hy + hz * hz;
+    return l;
+}
+template <class T>
+WDM_FI double eval_V_line(const VModel& vm, const LineV& l, double x, unsigned idx, cx<T> psi) {
+    double V = 0.0;
+    if (vm.flags & V_STATIC) V += vm.vstatic[idx];
+    if (vm.flags & V_HERNQUIST) {
+        const double dx = x - vm.hq_c[0];
+        V -= vm.hq_GM / (__builtin_sqrt(fmaT(dx, dx, l.hr2)) + vm.hq_a);
+    }
+    if (vm.flags & V_QUADRUPOLE) {
+        const double dx = x - vm.q_c[0];
+        V += fmaT(fmaT(l.qa, dx, l.qb), dx, l.qc);
+    }
+    if (vm.flags & V_KAPPA) V += vm.kappa * double(norm2(psi));
+    return V;
+}
+
 // ------------------------------------------------------------------ phases
 // Reduce theta to r in [-pi/4, pi/4] and quadrant q (theta = q*pi/2 + r), FP64.
 WDM_FI double reduce_quadrant(double theta, int& q) {
@@ -362,7 +396,7 @@ void k_x_kick(cx<T>* __restrict__ psi, const cx<T>* __restrict__ phihat, int nli
         WDM_UNROLL
         for (int kq = 0; kq < N1; ++kq) phi[kq] = T(0);
     }
-    const double yc = ka.ys[y], zc = ka.zs[z];
+    const LineV lv = line_V(ka.vm, ka.ys[y], ka.zs[z]);
     // psi loads only after Phi is done (otherwise the 48 loads are hoisted
     // above the C2R transform and both sets of registers are live at once)
     WDM_SCHED_FENCE();
@@ -375,7 +409,7 @@ void k_x_kick(cx<T>* __restrict__ psi, const cx<T>* __restrict__ phihat, int nli
         double V = 0.0;
         if (ka.vm.flags) {
             const double tx = double(x) * ka.xinv;
-            V = eval_V<T>(ka.vm, (1.0 - tx) * ka.x0 + tx * ka.x1, yc, zc, unsigned(lc) * N + x, v[n1]);
+            V = eval_V_line<T>(ka.vm, lv, (1.0 - tx) * ka.x0 + tx * ka.x1, unsigned(lc) * N + x, v[n1]);
         }
         T ph = phi[n1];
         WDM_OPAQUE(ph);

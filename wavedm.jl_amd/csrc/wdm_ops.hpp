@@ -28,10 +28,18 @@ inline LaunchEvents& launch_events() {
 
 namespace wdm {
 
-#ifndef WDM_NW
-#define WDM_NW 4
+// Waves per workgroup of the pass kernels, per precision.  Measured at 384^3
+// on MI355X: FP32 (<= 256 registers per lane) gains from 8-wave groups, whose
+// 64 consecutive columns make 512-byte row segments in the strided passes;
+// FP64 kernels (256 VGPRs + AGPRs, one wave per SIMD) do best with 2.
+#ifndef WDM_NW_F32
+#define WDM_NW_F32 8
 #endif
-constexpr int kNW = WDM_NW;  // waves per workgroup for the pass kernels
+#ifndef WDM_NW_F64
+#define WDM_NW_F64 2
+#endif
+template <class T>
+constexpr int kNW = sizeof(T) == 4 ? WDM_NW_F32 : WDM_NW_F64;
 
 template <class T>
 struct AxisOps {
@@ -51,15 +59,15 @@ struct AxisOps {
 };
 
 // Grid sizes (in workgroups of kNW waves).
-template <class F>
+template <class T, class F>
 inline unsigned x_blocks(int nlines) {
     const int waves = (nlines + F::LPW - 1) / F::LPW;
-    return unsigned((waves + kNW - 1) / kNW);
+    return unsigned((waves + kNW<T> - 1) / kNW<T>);
 }
-template <class F>
+template <class T, class F>
 inline unsigned col_blocks(int ncols, int nother) {
     const long waves = long((ncols + F::LPW - 1) / F::LPW) * nother;
-    return unsigned((waves + kNW - 1) / kNW);
+    return unsigned((waves + kNW<T> - 1) / kNW<T>);
 }
 
 template <class T, class F>
@@ -70,14 +78,14 @@ struct Launch {
     template <int MODE, int KF>
     static void col_t(cx<T>* d, int ncols, int nother, unsigned sl, unsigned so, const TwE* tw, KTabs kt,
                       hipStream_t s) {
-        WDM_LAUNCH((k_col<T, PsiP, F, MODE, KF, kNW>), dim3(col_blocks<F>(ncols, nother)), dim3(kNW * 64), s, d,
+        WDM_LAUNCH((k_col<T, PsiP, F, MODE, KF, kNW<T>>), dim3(col_blocks<T, F>(ncols, nother)), dim3(kNW<T> * 64), s, d,
                    ncols, nother, sl, so, tw, kt);
     }
     typedef PoissonPolicy PoisP;
     template <int MODE, int KF>
     static void colp_t(cx<T>* d, int ncols, int nother, unsigned sl, unsigned so, const TwE* tw, KTabs kt,
                        hipStream_t s) {
-        WDM_LAUNCH((k_col<T, PoisP, F, MODE, KF, kNW>), dim3(col_blocks<F>(ncols, nother)), dim3(kNW * 64), s, d,
+        WDM_LAUNCH((k_col<T, PoisP, F, MODE, KF, kNW<T>>), dim3(col_blocks<T, F>(ncols, nother)), dim3(kNW<T> * 64), s, d,
                    ncols, nother, sl, so, tw, kt);
     }
     // mode/kf combinations used by the plan; KF_POISSON passes run in Plain.
@@ -102,36 +110,36 @@ struct Launch {
     static void x_kick(bool fwd, cx<T>* psi, const cx<T>* ph, int nlines, int hp, const TwE* tw, const KickArgs& ka,
                        hipStream_t s) {
         if (fwd)
-            WDM_LAUNCH((k_x_kick<T, PsiP, F, true, kNW>), dim3(x_blocks<F>(nlines)), dim3(kNW * 64), s, psi, ph,
+            WDM_LAUNCH((k_x_kick<T, PsiP, F, true, kNW<T>>), dim3(x_blocks<T, F>(nlines)), dim3(kNW<T> * 64), s, psi, ph,
                        nlines, hp, tw, ka);
         else
-            WDM_LAUNCH((k_x_kick<T, PsiP, F, false, kNW>), dim3(x_blocks<F>(nlines)), dim3(kNW * 64), s, psi, ph,
+            WDM_LAUNCH((k_x_kick<T, PsiP, F, false, kNW<T>>), dim3(x_blocks<T, F>(nlines)), dim3(kNW<T> * 64), s, psi, ph,
                        nlines, hp, tw, ka);
     }
     static void x_inv_dens(cx<T>* psi, cx<T>* rh, int nlines, int hp, const TwE* tw, const double* border,
                            double* partials, hipStream_t s) {
-        WDM_LAUNCH((k_x_inv_dens<T, PsiP, F, kNW>), dim3(x_blocks<F>(nlines)), dim3(kNW * 64), s, psi, rh, nlines,
+        WDM_LAUNCH((k_x_inv_dens<T, PsiP, F, kNW<T>>), dim3(x_blocks<T, F>(nlines)), dim3(kNW<T> * 64), s, psi, rh, nlines,
                    hp, tw, border, partials);
     }
     static void x_dens(const cx<T>* psi, cx<T>* rh, int nlines, int hp, const TwE* tw, double* partials,
                        hipStream_t s) {
-        WDM_LAUNCH((k_x_dens<T, F, kNW>), dim3(x_blocks<F>(nlines)), dim3(kNW * 64), s, psi, rh, nlines, hp, tw,
+        WDM_LAUNCH((k_x_dens<T, F, kNW<T>>), dim3(x_blocks<T, F>(nlines)), dim3(kNW<T> * 64), s, psi, rh, nlines, hp, tw,
                    partials);
     }
     static void x_r2c_real(const double* src, cx<T>* rh, int nlines, int hp, const TwE* tw, hipStream_t s) {
-        WDM_LAUNCH((k_x_r2c_real<T, F, kNW>), dim3(x_blocks<F>(nlines)), dim3(kNW * 64), s, src, rh, nlines, hp,
+        WDM_LAUNCH((k_x_r2c_real<T, F, kNW<T>>), dim3(x_blocks<T, F>(nlines)), dim3(kNW<T> * 64), s, src, rh, nlines, hp,
                    tw);
     }
     static void x_c2r_real(const cx<T>* h, double* dst, int nlines, int hp, const TwE* tw, hipStream_t s) {
-        WDM_LAUNCH((k_x_c2r_real<T, F, kNW>), dim3(x_blocks<F>(nlines)), dim3(kNW * 64), s, h, dst, nlines, hp,
+        WDM_LAUNCH((k_x_c2r_real<T, F, kNW<T>>), dim3(x_blocks<T, F>(nlines)), dim3(kNW<T> * 64), s, h, dst, nlines, hp,
                    tw);
     }
     static void x_fft(int dir, cx<T>* psi, int nlines, const TwE* tw, hipStream_t s) {
         if (dir < 0)
-            WDM_LAUNCH((k_x_fft<T, PsiP, F, -1, kNW>), dim3(x_blocks<F>(nlines)), dim3(kNW * 64), s, psi, nlines,
+            WDM_LAUNCH((k_x_fft<T, PsiP, F, -1, kNW<T>>), dim3(x_blocks<T, F>(nlines)), dim3(kNW<T> * 64), s, psi, nlines,
                        tw);
         else
-            WDM_LAUNCH((k_x_fft<T, PsiP, F, +1, kNW>), dim3(x_blocks<F>(nlines)), dim3(kNW * 64), s, psi, nlines,
+            WDM_LAUNCH((k_x_fft<T, PsiP, F, +1, kNW<T>>), dim3(x_blocks<T, F>(nlines)), dim3(kNW<T> * 64), s, psi, nlines,
                        tw);
     }
     static const AxisOps<T>* table() {

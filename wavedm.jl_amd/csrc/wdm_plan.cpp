@@ -852,11 +852,11 @@ void init_plan(wdm_plan* p) {
     p->psi.get<cx<T>>(p->ncell);
     p->hpitch = ((p->nx / 2 + 1) + 7) / 8 * 8;
     p->rhohat.get<cx<T>>(size_t(p->hpitch) * p->ny * p->nz);
-    // partial-sum slots: one pair per wave of the x passes (upper bound)
-    // exactly the waves an x pass launches; every launched wave writes its pair
+    // partial-sum slots: one pair per launched wave of the x passes (every
+    // launched wave writes its pair, including the idle tail of the last group)
     const AxisOps<T>* ox = ops_of<T>(p, 0);
     const size_t waves = (size_t(p->ny) * p->nz + ox->lpw - 1) / ox->lpw;
-    p->npartials = 2 * ((waves + kNW - 1) / kNW) * kNW;
+    p->npartials = 2 * ((waves + kNW<T> - 1) / kNW<T>) * kNW<T>;
     p->partials.get<double>(std::max<size_t>(p->npartials, 2 * 1024));
     p->redout.get<double>(2);
 }
