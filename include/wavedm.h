@@ -160,6 +160,28 @@ int wdm_step_timed(wdm_plan* plan, int nsteps, double dt, double dt_kick, double
    function on the host grid (src/physics/schrodinger.jl:37-46). */
 int wdm_ic_solitons(wdm_plan* plan, const double* params, int nsol);
 
+/* Adjoint SSFM (reference split).  No counterpart in the reference, which
+   imports Zygote (src/WaveDM.jl:42) but defines no rule for the step; this is
+   what a ChainRulesCore rrule around wdm_step calls.
+
+   wdm_adjoint_forward: nsteps fused steps from the current psi (as wdm_step
+   with split WDM_SPLIT_REFERENCE), keeping a checkpoint of psi every
+   ckpt_every steps on the device.
+   wdm_adjoint_backward: given grad_psi = dL/dRe(psi_N) + i dL/dIm(psi_N) on
+   the host (dtype WDM_C128/WDM_C64), returns grad_psi0 (same packing and
+   dtype; NULL to skip) and grad_params[0..nparams):
+     0..4  dL/dM, dL/da, dL/dcx, dL/dcy, dL/dcz of the Hernquist model
+     5     dL/dsigma for dt -> sigma dt, dt_kick -> sigma dt_kick at sigma = 1
+           (only with WDM_ADJ_DT_SCALE, else NaN; needs a time-independent V)
+   Segments of ckpt_every steps are recomputed during the backward sweep.
+   Self-gravity, static V, Hernquist, quadrupole(t) and kappa|psi|^2 are
+   differentiated; psi is left at psi_N. */
+#define WDM_ADJ_DT_SCALE 1u
+#define WDM_ADJ_NPARAM 6
+int wdm_adjoint_forward(wdm_plan* plan, int nsteps, double dt, double dt_kick, double t0, int ckpt_every);
+int wdm_adjoint_backward(wdm_plan* plan, const void* grad_psi, int dtype, void* grad_psi0, double* grad_params,
+                         int nparams, unsigned flags);
+
 #ifdef __cplusplus
 }
 #endif

@@ -41,6 +41,8 @@ WDM_RED_VEXT = 3
 WDM_RED_PSIMAX = 4
 WDM_RED_PHIMAX = 5
 WDM_RED_COUNT = 6
+WDM_ADJ_DT_SCALE = 1
+WDM_ADJ_NPARAM = 6
 
 # every symbol include/wavedm.h declares: name -> (restype, argtypes)
 _P = C.c_void_p
@@ -72,6 +74,8 @@ SIGNATURES = {
     "wdm_synchronize": (_I, [_P]),
     "wdm_step_timed": (_I, [_P, _I, _D, _D, _D, _DP, _I, _DP]),
     "wdm_ic_solitons": (_I, [_P, _DP, _I]),
+    "wdm_adjoint_forward": (_I, [_P, _I, _D, _D, _D, _I]),
+    "wdm_adjoint_backward": (_I, [_P, _P, _I, _P, _DP, _I, C.c_uint]),
 }
 PASS_NAMES = ["x_kick_fwd", "y_fwd", "z_drift", "y_inv", "x_inv_density", "poisson_y_fwd", "poisson_z",
               "poisson_y_inv"]

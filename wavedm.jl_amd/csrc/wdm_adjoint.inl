@@ -1,0 +1,424 @@
+// Adjoint SSFM (included at the end of wdm_plan.cpp; shares its helpers).
+//
+// Forward map of one reference-split step (src/simulation_main.jl:536-547):
+//   psi' = E psi_n,  E = exp(-i tau (Phi[psi_n] + V)),  tau = (m/hbar) dt_kick
+//   psi_{n+1} = B U psi',  U = F^-1 P F,  P = exp(-i (hbar/m) k^2 dt / 2)
+// with Phi[psi] = 4 pi G lap^-1 (|psi|^2 - <|psi|^2>) (self-adjoint, real).
+//
+// Gradients are packed g = dL/dRe(psi) + i dL/dIm(psi), so dL = Re sum conj(g) dpsi.
+// Backward through one step, given g_{n+1}:
+//   g'  = U^H (B g_{n+1})                         (conjugate drift)
+//   s   = tau Im(conj(g') psi')                   (real field)
+//   u   = Phi-operator(s) + kappa s               (linearised Poisson / kappa|psi|^2)
+//   g_n = conj(E) g' + 2 u psi_n
+//   dL/dtheta += sum s dV/dtheta                  (Hernquist M, a, centre)
+//   dL/dsigma += sum s (Phi + V) + (1/N) sum_k w_k Im(conj(F g')_k (F psi')_k),
+//                w_k = (hbar/m) k^2 dt / 2        (sigma scales dt and dt_kick)
+// Memory: checkpoints of psi every K steps; each segment is recomputed with the
+// fused forward step while psi_n and Phi_n are taped, then walked backwards.
+// No counterpart in the reference (it imports Zygote, src/WaveDM.jl:42, but
+// defines no rule for the step).
+
+namespace {
+
+// ------------------------------------------------------------------ kernels
+WDM_FI void cell_xyz(size_t i, int nx, int ny, int& x, int& y, int& z) {
+    x = int(i % size_t(nx));
+    const size_t r = i / size_t(nx);
+    y = int(r % size_t(ny));
+    z = int(r / size_t(ny));
+}
+
+WDM_FI cx<double> expmi(double theta) {
+    int q;
+    const double r = reduce_quadrant(theta, q);
+    double s, c;
+    sincos_small(r, s, c);
+    return rot_quadrant(mk(c, -s), q);  // exp(-i theta)
+}
+
+// deterministic block sum of NV values per thread into partials[NV*block + j]
+template <int NV>
+WDM_FI void block_sums(double (&acc)[NV], double (*sh)[256], double* __restrict__ partials) {
+    const int t = int(threadIdx.x);
+    for (int j = 0; j < NV; ++j) sh[j][t] = acc[j];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w)
+            for (int j = 0; j < NV; ++j) sh[j][t] += sh[j][t + w];
+        __syncthreads();
+    }
+    if (t == 0)
+        for (int j = 0; j < NV; ++j) partials[NV * blockIdx.x + j] = sh[j][0];
+}
+
+constexpr int kAdjNV = 6;
+
+// s = tau Im(conj(g') E psi) and the parameter partial sums.
+template <class T>
+WDM_KERNEL(256)
+void k_adj_s(const cx<T>* __restrict__ g, const cx<T>* __restrict__ psi, const double* __restrict__ phi,
+             double* __restrict__ s, int nx, int ny, int nz, const double* __restrict__ xs,
+             const double* __restrict__ ys, const double* __restrict__ zs, VModel vm, double tau,
+             double* __restrict__ partials) {
+    WDM_SHARED double sh[kAdjNV][256];
+    double acc[kAdjNV] = {0, 0, 0, 0, 0, 0};
+    const size_t n = size_t(nx) * ny * nz;
+    for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) {
+        int x, y, z;
+        cell_xyz(i, nx, ny, x, y, z);
+        const cx<T> pv = psi[i];
+        const double V = vm.flags ? eval_V<T>(vm, xs[x], ys[y], zs[z], unsigned(i), pv) : 0.0;
+        const double ph = (phi ? phi[i] : 0.0) + V;
+        const cx<double> e = expmi(tau * ph);
+        const double pr = double(pv.x), pi = double(pv.y);
+        const double qr = e.x * pr - e.y * pi, qi = e.x * pi + e.y * pr;  // psi'
+        const double gr = double(g[i].x), gi = double(g[i].y);
+        const double sv = tau * (gr * qi - gi * qr);
+        s[i] = sv;
+        if (vm.flags & V_HERNQUIST) {
+            const double dx = xs[x] - vm.hq_c[0], dy = ys[y] - vm.hq_c[1], dz = zs[z] - vm.hq_c[2];
+            const double r = __builtin_sqrt(dx * dx + dy * dy + dz * dz);
+            const double ra = r + vm.hq_a;
+            const double G = vm.hq_G;
+            acc[0] += sv * (-G / ra);
+            const double dVdr = vm.hq_GM / (ra * ra);
+            acc[1] += sv * dVdr;
+            if (r > 0.0) {
+                acc[2] -= sv * dVdr * dx / r;
+                acc[3] -= sv * dVdr * dy / r;
+                acc[4] -= sv * dVdr * dz / r;
+            }
+        }
+        acc[5] += sv * ph;
+    }
+    block_sums<kAdjNV>(acc, sh, partials);
+}
+
+// g = conj(E) g' + 2 (u + kappa s) psi  (then * border along x when given)
+template <class T>
+WDM_KERNEL(256)
+void k_adj_g(cx<T>* __restrict__ g, const cx<T>* __restrict__ psi, const double* __restrict__ phi,
+             const double* __restrict__ u, const double* __restrict__ s, int nx, int ny, int nz,
+             const double* __restrict__ xs, const double* __restrict__ ys, const double* __restrict__ zs, VModel vm,
+             double tau, const double* __restrict__ border) {
+    const size_t n = size_t(nx) * ny * nz;
+    for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) {
+        int x, y, z;
+        cell_xyz(i, nx, ny, x, y, z);
+        const cx<T> pv = psi[i];
+        const double V = vm.flags ? eval_V<T>(vm, xs[x], ys[y], zs[z], unsigned(i), pv) : 0.0;
+        const cx<double> e = expmi(tau * ((phi ? phi[i] : 0.0) + V));
+        const double gr = double(g[i].x), gi = double(g[i].y);
+        double w = u ? u[i] : 0.0;
+        if (vm.flags & V_KAPPA) w += vm.kappa * s[i];
+        // e = E = exp(-i theta): conj(E) g' = (e.x - i e.y)(gr + i gi)
+        double rr = e.x * gr + e.y * gi + 2.0 * w * double(pv.x);
+        double ri = e.x * gi - e.y * gr + 2.0 * w * double(pv.y);
+        if (border) {
+            rr *= border[x];
+            ri *= border[x];
+        }
+        g[i] = mk(T(rr), T(ri));
+    }
+}
+
+// psi' = E psi (real space), for the dt-scale term
+template <class T>
+WDM_KERNEL(256)
+void k_adj_kick(cx<T>* __restrict__ out, const cx<T>* __restrict__ psi, const double* __restrict__ phi, int nx,
+                int ny, int nz, const double* __restrict__ xs, const double* __restrict__ ys,
+                const double* __restrict__ zs, VModel vm, double tau) {
+    const size_t n = size_t(nx) * ny * nz;
+    for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) {
+        int x, y, z;
+        cell_xyz(i, nx, ny, x, y, z);
+        const cx<T> pv = psi[i];
+        const double V = vm.flags ? eval_V<T>(vm, xs[x], ys[y], zs[z], unsigned(i), pv) : 0.0;
+        const cx<double> e = expmi(tau * ((phi ? phi[i] : 0.0) + V));
+        const double pr = double(pv.x), pi = double(pv.y);
+        out[i] = mk(T(e.x * pr - e.y * pi), T(e.x * pi + e.y * pr));
+    }
+}
+
+// k space: G <- conj(P) G / N and partial sums of w_k Im(conj(G_new) Psi') / N
+template <class T>
+WDM_KERNEL(256)
+void k_adj_spec(cx<T>* __restrict__ G, const cx<T>* __restrict__ Psi, int nx, int ny, int nz,
+                const double* __restrict__ kx2, const double* __restrict__ ky2, const double* __restrict__ kz2,
+                double c /* (hbar/m) dt / 2 */, double inv_n, double* __restrict__ partials) {
+    WDM_SHARED double sh[1][256];
+    double acc[1] = {0.0};
+    const size_t n = size_t(nx) * ny * nz;
+    for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) {
+        int x, y, z;
+        cell_xyz(i, nx, ny, x, y, z);
+        const double w = c * (kx2[x] + ky2[y] + kz2[z]);
+        const cx<double> e = expmi(w);  // P; conj(P) = (e.x, -e.y)
+        const double gr = double(G[i].x), gi = double(G[i].y);
+        const double hr = (e.x * gr + e.y * gi) * inv_n, hi = (e.x * gi - e.y * gr) * inv_n;
+        G[i] = mk(T(hr), T(hi));
+        const double ar = double(Psi[i].x), ai = double(Psi[i].y);
+        acc[0] += w * (hr * ai - hi * ar);  // Im(conj(h) Psi) ; the 1/N of F g' is in h
+    }
+    block_sums<1>(acc, sh, partials);
+}
+
+// acc[j] += sum_b partials[NV*b + j], fixed order (one thread per component)
+WDM_KERNEL(64)
+void k_adj_accumulate(const double* __restrict__ partials, int nb, int nv, double* __restrict__ acc) {
+    const int j = int(threadIdx.x);
+    if (j >= nv) return;
+    double sum = 0.0, comp = 0.0;
+    for (int b = 0; b < nb; ++b) {
+        const double v = partials[nv * b + j];
+        const double t = sum + v;
+        comp += (std::fabs(sum) >= std::fabs(v)) ? (sum - t) + v : (v - t) + sum;
+        sum = t;
+    }
+    acc[j] += sum + comp;
+}
+
+WDM_KERNEL(64)
+void k_adj_zero(double* __restrict__ acc, int n) {
+    const int j = int(threadIdx.x);
+    if (j < n) acc[j] = 0.0;
+}
+
+// ------------------------------------------------------------------ host side
+struct AdjState {
+    bool ready = false;
+    int nsteps = 0, K = 0, nckpt = 0;
+    double dt = 0.0, dt_kick = 0.0, t0 = 0.0;
+    DBuf ckpt;   // (nckpt + 1) psi: psi_{kK}, k < nckpt, then psi_N
+    DBuf tpsi;   // K psi of the current segment
+    DBuf tphi;   // K real Phi (FP64) of the current segment
+    DBuf g, s, u, spec;  // adjoint field, real fields, k-space scratch (dt-scale)
+    DBuf part, acc;
+    DBuf kfull[3];
+};
+
+void adj_release(wdm_plan* p) {
+    if (!p->adj) return;
+    AdjState& A = *p->adj;
+    for (DBuf* b : {&A.ckpt, &A.tpsi, &A.tphi, &A.g, &A.s, &A.u, &A.spec, &A.part, &A.acc, &A.kfull[0], &A.kfull[1],
+                    &A.kfull[2]})
+        b->reset();
+    delete p->adj;
+    p->adj = nullptr;
+}
+
+AdjState& adj_of(wdm_plan* p) {
+    if (!p->adj) p->adj = new AdjState();
+    return *p->adj;
+}
+
+unsigned adj_blocks(size_t n) { return unsigned(std::min<size_t>((n + 255) / 256, 2048)); }
+
+template <class T>
+void adj_forward(wdm_plan* p, int nsteps, double dt, double dt_kick, double t0, int K) {
+    if (p->kspace) fail(WDM_ERR_STATE, "wdm_adjoint_forward: psi is in k-space");
+    AdjState& A = adj_of(p);
+    A.ready = false;
+    const size_t n = p->ncell;
+    A.nsteps = nsteps;
+    A.K = K;
+    A.nckpt = (nsteps + K - 1) / K;
+    A.dt = dt;
+    A.dt_kick = dt_kick;
+    A.t0 = t0;
+    cx<T>* ck = A.ckpt.get<cx<T>>(n * size_t(A.nckpt + 1));
+    for (int k = 0; k < A.nckpt; ++k) {
+        hip_check(hipMemcpyAsync(ck + n * size_t(k), p->psi.as<cx<T>>(), sizeof(cx<T>) * n, hipMemcpyDeviceToDevice,
+                                 p->stream),
+                  "checkpoint");
+        const int len = std::min(K, nsteps - k * K);
+        do_step<T>(p, len, dt, dt_kick, t0 + double(k) * K * dt, WDM_SPLIT_REFERENCE);
+    }
+    hip_check(hipMemcpyAsync(ck + n * size_t(A.nckpt), p->psi.as<cx<T>>(), sizeof(cx<T>) * n,
+                             hipMemcpyDeviceToDevice, p->stream),
+              "final state");
+    hip_check(hipStreamSynchronize(p->stream), "adjoint forward");
+    A.ready = true;
+}
+
+// g (device) <- U^H g, the conjugate drift; with Psi given, also the dt-scale
+// spectral term into acc[5] (unfused passes)
+template <class T>
+void adj_conj_drift(wdm_plan* p, cx<T>* g, const cx<T>* Psi, double* part, double* acc) {
+    typedef typename TwEntry<T>::type TwE;
+    AdjState& A = adj_of(p);
+    const AxisOps<T>* ox = ops_of<T>(p, 0);
+    const AxisOps<T>* oy = ops_of<T>(p, 1);
+    const AxisOps<T>* oz = ops_of<T>(p, 2);
+    KTabs none = {nullptr, nullptr, nullptr, 0.0};
+    const int nlines = p->ny * p->nz;
+    ox->x_fft(-1, g, nlines, p->ax[0].tw.as<TwE>(), p->stream);
+    if (p->ny > 1)
+        oy->col(COL_FWD, KF_NONE, g, p->nx, p->nz, unsigned(p->nx), unsigned(p->nx * p->ny), p->ax[1].tw.as<TwE>(),
+                none, p->stream);
+    if (!Psi) {
+        ensure_drift(p, A.dt);
+        KTabs kt = {p->ax[0].drift_conj.as<double>(), p->ax[1].drift_conj.as<double>(),
+                    p->ax[2].drift_conj.as<double>(), 1.0 / double(p->ncell)};
+        oz->col(COL_FMI, KF_NONE, g, p->nx, p->ny, unsigned(p->nx * p->ny), unsigned(p->nx), p->ax[2].tw.as<TwE>(), kt,
+                p->stream);
+    } else {
+        if (p->nz > 1)
+            oz->col(COL_FWD, KF_NONE, g, p->nx, p->ny, unsigned(p->nx * p->ny), unsigned(p->nx),
+                    p->ax[2].tw.as<TwE>(), none, p->stream);
+        const unsigned nb = adj_blocks(p->ncell);
+        WDM_LAUNCH((k_adj_spec<T>), dim3(nb), dim3(256), p->stream, g, Psi, p->nx, p->ny, p->nz,
+                   A.kfull[0].as<double>(), A.kfull[1].as<double>(), A.kfull[2].as<double>(),
+                   0.5 * p->hbar_over_m * A.dt, 1.0 / double(p->ncell), part);
+        WDM_LAUNCH((k_adj_accumulate), dim3(1), dim3(64), p->stream, (const double*)part, int(nb), 1, acc + 5);
+        if (p->nz > 1)
+            oz->col(COL_INV, KF_NONE, g, p->nx, p->ny, unsigned(p->nx * p->ny), unsigned(p->nx),
+                    p->ax[2].tw.as<TwE>(), none, p->stream);
+    }
+    if (p->ny > 1)
+        oy->col(COL_INV, KF_NONE, g, p->nx, p->nz, unsigned(p->nx), unsigned(p->nx * p->ny), p->ax[1].tw.as<TwE>(),
+                none, p->stream);
+    ox->x_fft(+1, g, nlines, p->ax[0].tw.as<TwE>(), p->stream);
+    hip_check(hipGetLastError(), "conjugate drift");
+}
+
+template <class T>
+void adj_backward(wdm_plan* p, const void* grad, int dtype, void* grad0, double* gparams, int nparams,
+                  unsigned flags) {
+    typedef typename TwEntry<T>::type TwE;
+    AdjState& A = adj_of(p);
+    if (!A.ready) fail(WDM_ERR_STATE, "wdm_adjoint_backward: no taped forward run (call wdm_adjoint_forward)");
+    const bool want_dt = (flags & WDM_ADJ_DT_SCALE) != 0;
+    if (want_dt && (p->vflags & V_QUADRUPOLE) && p->quad[9] != 0.0)
+        fail(WDM_ERR_UNSUPPORTED, "wdm_adjoint_backward: dt-scale gradient needs a time-independent potential");
+    const size_t n = p->ncell;
+    const AxisOps<T>* ox = ops_of<T>(p, 0);
+    const int nlines = p->ny * p->nz;
+    const unsigned nb = adj_blocks(n);
+    cx<T>* ck = A.ckpt.as<cx<T>>();
+    cx<T>* tpsi = A.tpsi.get<cx<T>>(n * size_t(A.K));
+    double* tphi = p->self_gravity ? A.tphi.get<double>(n * size_t(A.K)) : nullptr;
+    cx<T>* g = A.g.get<cx<T>>(n);
+    double* s = A.s.get<double>(n);
+    double* u = p->self_gravity ? A.u.get<double>(n) : nullptr;
+    double* part = A.part.get<double>(size_t(kAdjNV) * nb);
+    double* acc = A.acc.get<double>(kAdjNV);
+    cx<T>* spec = want_dt ? A.spec.get<cx<T>>(n) : nullptr;
+    if (want_dt) {
+        for (int a = 0; a < 3; ++a) {
+            const int na = p->ax[a].n;
+            std::vector<double> h(na);
+            for (int m = 0; m < na; ++m) {
+                const double k = drift_k(na, p->X[a], m);
+                h[m] = k * k;
+            }
+            upload(A.kfull[a], h);
+        }
+    }
+    WDM_LAUNCH((k_adj_zero), dim3(1), dim3(64), p->stream, acc, kAdjNV);
+    // g_N (then * border: the first backward step starts from B g_N)
+    set_psi_dev<T>(p, grad, dtype, true);  // borrow psi as the upload target
+    hip_check(hipMemcpyAsync(g, p->psi.as<cx<T>>(), sizeof(cx<T>) * n, hipMemcpyDeviceToDevice, p->stream), "g copy");
+    const double* border = p->border_on ? p->border.as<double>() : nullptr;
+    const double* xs = p->ax[0].coord.as<double>();
+    const double* ys = p->ax[1].coord.as<double>();
+    const double* zs = p->ax[2].coord.as<double>();
+    if (border) {
+        // B g_N: reuse k_adj_g with E = 1, u = 0 (tau = 0, no potential)
+        VModel none;
+        std::memset(&none, 0, sizeof(none));
+        WDM_LAUNCH((k_adj_g<T>), dim3(nb), dim3(256), p->stream, g, (const cx<T>*)p->psi.as<cx<T>>(),
+                   (const double*)nullptr, (const double*)nullptr, (const double*)s, p->nx, p->ny, p->nz, xs, ys, zs,
+                   none, 0.0, border);
+    }
+    const double tau = A.dt_kick / p->hbar_over_m;
+    for (int k = A.nckpt - 1; k >= 0; --k) {
+        const int n0 = k * A.K;
+        const int len = std::min(A.K, A.nsteps - n0);
+        // recompute the segment, taping psi_n and Phi_n
+        hip_check(hipMemcpyAsync(p->psi.as<cx<T>>(), ck + n * size_t(k), sizeof(cx<T>) * n, hipMemcpyDeviceToDevice,
+                                 p->stream),
+                  "restore checkpoint");
+        p->kspace = false;
+        p->phi_valid = false;
+        p->mass_valid = false;
+        for (int i = 0; i < len; ++i) {
+            ensure_phi<T>(p);
+            hip_check(hipMemcpyAsync(tpsi + n * size_t(i), p->psi.as<cx<T>>(), sizeof(cx<T>) * n,
+                                     hipMemcpyDeviceToDevice, p->stream),
+                      "tape psi");
+            if (tphi)
+                ox->x_c2r_real(p->rhohat.as<cx<T>>(), tphi + n * size_t(i), nlines, p->hpitch, p->ax[0].tw.as<TwE>(),
+                               p->stream);
+            if (i + 1 < len) do_step<T>(p, 1, A.dt, A.dt_kick, A.t0 + double(n0 + i) * A.dt, WDM_SPLIT_REFERENCE);
+        }
+        // walk the segment backwards
+        for (int i = len - 1; i >= 0; --i) {
+            const int step = n0 + i;
+            const cx<T>* pn = tpsi + n * size_t(i);
+            const double* fn = tphi ? tphi + n * size_t(i) : nullptr;
+            const VModel vm = make_vmodel(p, A.t0 + double(step) * A.dt);
+            if (want_dt) {
+                WDM_LAUNCH((k_adj_kick<T>), dim3(nb), dim3(256), p->stream, spec, pn, fn, p->nx, p->ny, p->nz, xs, ys,
+                           zs, vm, tau);
+                forward3d<T>(p, spec);
+            }
+            adj_conj_drift<T>(p, g, spec, part, acc);
+            WDM_LAUNCH((k_adj_s<T>), dim3(nb), dim3(256), p->stream, (const cx<T>*)g, pn, fn, s, p->nx, p->ny, p->nz,
+                       xs, ys, zs, vm, tau, part);
+            WDM_LAUNCH((k_adj_accumulate), dim3(1), dim3(64), p->stream, (const double*)part, int(nb), kAdjNV, acc);
+            if (u) {
+                ox->x_r2c_real(s, p->rhohat.as<cx<T>>(), nlines, p->hpitch, p->ax[0].tw.as<TwE>(), p->stream);
+                poisson_chain<T>(p, p->four_pi_G);
+                ox->x_c2r_real(p->rhohat.as<cx<T>>(), u, nlines, p->hpitch, p->ax[0].tw.as<TwE>(), p->stream);
+            }
+            WDM_LAUNCH((k_adj_g<T>), dim3(nb), dim3(256), p->stream, g, pn, fn, (const double*)u, (const double*)s,
+                       p->nx, p->ny, p->nz, xs, ys, zs, vm, tau, step > 0 ? border : (const double*)nullptr);
+            hip_check(hipGetLastError(), "adjoint step");
+        }
+    }
+    // outputs; restore psi_N
+    double h[kAdjNV];
+    hip_check(hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, p->stream), "param grads");
+    hip_check(hipMemcpyAsync(p->psi.as<cx<T>>(), ck + n * size_t(A.nckpt), sizeof(cx<T>) * n, hipMemcpyDeviceToDevice,
+                             p->stream),
+              "restore final state");
+    hip_check(hipStreamSynchronize(p->stream), "adjoint backward");
+    p->kspace = false;
+    p->phi_valid = false;
+    p->mass_valid = false;
+    if (grad0) {
+        // download g_0 through the same conversion as wdm_get_psi
+        cx<T>* keep = p->psi.as<cx<T>>();
+        p->psi.p = g;
+        get_psi_dev<T>(p, grad0, dtype, true);
+        p->psi.p = keep;
+    }
+    if (gparams) {
+        for (int j = 0; j < nparams && j < kAdjNV; ++j) gparams[j] = h[j];
+        if (nparams > 5 && !want_dt) gparams[5] = NAN;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int wdm_adjoint_forward(wdm_plan* p, int nsteps, double dt, double dt_kick, double t0, int ckpt_every) {
+    if (!p) return fail_code(nullptr, WDM_ERR_ARG, "plan is NULL");
+    if (nsteps < 1 || ckpt_every < 1 || !std::isfinite(dt) || !std::isfinite(dt_kick) || !std::isfinite(t0))
+        return fail_code(p, WDM_ERR_ARG, "wdm_adjoint_forward: bad nsteps, ckpt_every or time arguments");
+    return guarded(p, [&] { DISPATCH(p, adj_forward, p, nsteps, dt, dt_kick, t0, ckpt_every); });
+}
+
+int wdm_adjoint_backward(wdm_plan* p, const void* grad_psi, int dtype, void* grad_psi0, double* grad_params,
+                         int nparams, unsigned flags) {
+    if (!p || !grad_psi) return fail_code(p, WDM_ERR_ARG, "wdm_adjoint_backward: NULL argument");
+    if (dtype != WDM_C128 && dtype != WDM_C64) return fail_code(p, WDM_ERR_ARG, "wdm_adjoint_backward: bad dtype");
+    if (nparams < 0 || (nparams > 0 && !grad_params))
+        return fail_code(p, WDM_ERR_ARG, "wdm_adjoint_backward: bad grad_params");
+    return guarded(p, [&] { DISPATCH(p, adj_backward, p, grad_psi, dtype, grad_psi0, grad_params, nparams, flags); });
+}
+
+}  // extern "C"

@@ -77,6 +77,7 @@ struct VModel {
     int flags;
     const double* vstatic;   // full grid, x-fastest (V_STATIC)
     double hq_GM, hq_a, hq_c[3];  // V = -GM / (|x - c| + a)
+    double hq_G;             // G alone (dV/dM of the adjoint)
     double qT[6];            // Txx Tyy Tzz Txy Txz Tyz at the kick time; V = 1/2 (x-c)^T T (x-c)
     double q_c[3];
     double kappa;            // V = kappa |psi|^2 (GNLSE term)

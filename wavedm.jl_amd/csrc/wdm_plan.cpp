@@ -86,6 +86,7 @@ struct AxisState {
     DBuf tw;                    // twiddle table
     DBuf coord;                 // FP64 coordinates
     DBuf drift;                 // complex FP64 drift factors for dt_cached
+    DBuf drift_conj;            // their complex conjugates (adjoint)
     DBuf k2;                    // FP64 k^2 (FFT order; x axis: half spectrum)
 };
 
@@ -106,6 +107,11 @@ struct PassTimer {
         for (hipEvent_t e : ev) hipEventDestroy(e);
     }
 };
+
+namespace {
+struct AdjState;
+void adj_release(wdm_plan* p);
+}  // namespace
 
 struct wdm_plan {
     int nx, ny, nz;
@@ -135,6 +141,7 @@ struct wdm_plan {
     double kappa = 0.0;
     double last_kick_t = 0.0;
     PassTimer* timer = nullptr;
+    AdjState* adj = nullptr;  // adjoint tape (wdm_adjoint_*), created on first use
     std::string err;
 };
 
@@ -210,6 +217,8 @@ void ensure_drift(wdm_plan* p, double dt) {
             h[2 * m + 1] = -std::sin(th);
         }
         upload(p->ax[a].drift, h);
+        for (int m = 0; m < n; ++m) h[2 * m + 1] = -h[2 * m + 1];
+        upload(p->ax[a].drift_conj, h);
     }
     p->dt_cached = dt;
     p->hbar_cached = p->hbar_over_m;
@@ -307,6 +316,7 @@ VModel make_vmodel(const wdm_plan* p, double t) {
     vm.vstatic = p->vstatic.as<double>();
     const double G = p->four_pi_G / (4.0 * M_PI);
     vm.hq_GM = G * p->hq[0];
+    vm.hq_G = G;
     vm.hq_a = p->hq[1];
     vm.hq_c[0] = p->hq[2];
     vm.hq_c[1] = p->hq[3];
@@ -950,10 +960,12 @@ int wdm_plan_create(wdm_plan** out, int nx, int ny, int nz, double xmax, double 
 int wdm_plan_destroy(wdm_plan* p) {
     if (!p) return WDM_OK;
     if (p->stream) hipStreamSynchronize(p->stream);
+    adj_release(p);
     for (auto& a : p->ax) {
         a.tw.reset();
         a.coord.reset();
         a.drift.reset();
+        a.drift_conj.reset();
         a.k2.reset();
     }
     for (DBuf* b : {&p->psi, &p->rhohat, &p->partials, &p->redout, &p->border, &p->vstatic, &p->scratch,
@@ -1180,3 +1192,5 @@ int wdm_synchronize(wdm_plan* p) {
 }
 
 }  // extern "C"
+
+#include "wdm_adjoint.inl"

@@ -1,0 +1,205 @@
+# WaveDMHIP.jl -- Julia binding of libwavedm_hip.so (include/wavedm.h).
+#
+# A `:hip` backend for WaveDM.jl's solver step: the functions below keep the
+# names and argument meaning of the reference API and forward to the C ABI
+# with `ccall`.  Arrays cross as `pointer(A)` (column-major, x fastest,
+# interleaved complex), exactly the layout the library expects.
+#
+# Reference interfaces mirrored:
+#   apply_kick_step!                 src/solver/KDK.jl:37-77
+#   apply_drift_step!                src/solver/KDK.jl:88-101
+#   compute_gravitational_potential  src/solver/gravity.jl:7-35
+#   parallel_poisson                 src/parallel_poisson.jl:36-68
+#   compute_timestep                 src/physics/poisson.jl:35-50
+#   the loop body                    src/simulation_main.jl:536-551
+#
+# This file is not executed in this repository (the image has no Julia); the
+# Python harness wavedm.jl_amd/kdk.py issues the same C-ABI call sequence and
+# is what the tests run.
+module WaveDMHIP
+
+export HIPPlan, set_psi!, get_psi, set_border!, set_static_potential!, set_hernquist!,
+       set_quadrupole!, set_kappa!, parallel_poisson, compute_gravitational_potential,
+       apply_kick_step!, apply_drift_step!, step!, reduce_diagnostics, cfl_limits
+
+const libwdm = get(ENV, "WAVEDM_HIP_LIB", joinpath(@__DIR__, "..", "lib", "libwavedm_hip.so"))
+
+const WDM_F64 = Cint(0)
+const WDM_F32 = Cint(1)
+const WDM_C128 = Cint(0)
+const WDM_C64 = Cint(1)
+const WDM_GRID_CONSISTENT = Cuint(1)
+const WDM_SPLIT_REFERENCE = Cint(0)
+const WDM_SPLIT_STRANG = Cint(1)
+const WDM_V_HERNQUIST = Cint(2)
+const WDM_V_QUADRUPOLE = Cint(4)
+const WDM_V_KAPPA = Cint(8)
+const WDM_RED_COUNT = 6
+
+struct WaveDMHIPError <: Exception
+    code::Int
+    msg::String
+end
+Base.showerror(io::IO, e::WaveDMHIPError) = print(io, "libwavedm_hip error ", e.code, ": ", e.msg)
+
+function check(rc::Cint, h::Ptr{Cvoid})
+    if rc != 0
+        msg = unsafe_string(ccall((:wdm_last_error, libwdm), Cstring, (Ptr{Cvoid},), h))
+        throw(WaveDMHIPError(Int(rc), msg))
+    end
+    return nothing
+end
+
+"""
+    HIPPlan(Nx, Ny, Nz, Xmax, Ymax, Zmax; precision=:f64, consistent_grid=false,
+            hbar_over_m=1.0, four_pi_G=4π)
+
+Device state of one simulation box (ψ, spectra, Poisson buffers, k tables).
+The reference grid `x = LinRange(-Xmax, Xmax, Nx)` (Δ = 2Xmax/(Nx-1)) is the
+default; `consistent_grid=true` uses Δ = 2Xmax/Nx.  The reference's code units
+are `hbar_over_m = 1`, `four_pi_G = 4π`.
+"""
+mutable struct HIPPlan
+    h::Ptr{Cvoid}
+    dims::NTuple{3,Int}
+    precision::Symbol
+    function HIPPlan(Nx::Integer, Ny::Integer, Nz::Integer, Xmax::Real, Ymax::Real, Zmax::Real;
+                     precision::Symbol=:f64, consistent_grid::Bool=false,
+                     hbar_over_m::Real=1.0, four_pi_G::Real=4π)
+        r = Ref{Ptr{Cvoid}}(C_NULL)
+        prec = precision === :f32 ? WDM_F32 : WDM_F64
+        flags = consistent_grid ? WDM_GRID_CONSISTENT : Cuint(0)
+        rc = ccall((:wdm_plan_create, libwdm), Cint,
+                   (Ref{Ptr{Cvoid}}, Cint, Cint, Cint, Cdouble, Cdouble, Cdouble, Cint, Cuint),
+                   r, Nx, Ny, Nz, Xmax, Ymax, Zmax, prec, flags)
+        check(rc, C_NULL)
+        p = new(r[], (Int(Nx), Int(Ny), Int(Nz)), precision)
+        finalizer(destroy!, p)
+        check(ccall((:wdm_set_coeffs, libwdm), Cint, (Ptr{Cvoid}, Cdouble, Cdouble), p.h, hbar_over_m,
+                    four_pi_G), p.h)
+        return p
+    end
+end
+
+function destroy!(p::HIPPlan)
+    if p.h != C_NULL
+        ccall((:wdm_plan_destroy, libwdm), Cint, (Ptr{Cvoid},), p.h)
+        p.h = C_NULL
+    end
+    return nothing
+end
+
+_dtype(::Type{ComplexF64}) = WDM_C128
+_dtype(::Type{ComplexF32}) = WDM_C64
+
+function set_psi!(p::HIPPlan, ψ::Array{T,3}) where {T<:Union{ComplexF64,ComplexF32}}
+    size(ψ) == p.dims || throw(DimensionMismatch("ψ is $(size(ψ)), plan is $(p.dims)"))
+    check(ccall((:wdm_set_psi, libwdm), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cint), p.h, ψ, _dtype(T)), p.h)
+    return p
+end
+
+function get_psi(p::HIPPlan, ::Type{T}=ComplexF64) where {T<:Union{ComplexF64,ComplexF32}}
+    ψ = Array{T,3}(undef, p.dims)
+    check(ccall((:wdm_get_psi, libwdm), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cint), p.h, ψ, _dtype(T)), p.h)
+    return ψ
+end
+
+"Absorbing border along x (the reference's rank-1 `border`, src/simulation_main.jl:415)."
+function set_border!(p::HIPPlan, border::Union{Nothing,AbstractVector{<:Real}})
+    b = border === nothing ? C_NULL : Vector{Float64}(border)
+    check(ccall((:wdm_set_border, libwdm), Cint, (Ptr{Cvoid}, Ptr{Cdouble}, Cint), p.h, b, 1), p.h)
+    return p
+end
+
+"Static external potential on the grid (replaces `V.(xxx, yyy, zzz, ψ)` when V does not depend on t or ψ)."
+function set_static_potential!(p::HIPPlan, V::Union{Nothing,Array{<:Real,3}})
+    v = V === nothing ? C_NULL : Array{Float64,3}(V)
+    check(ccall((:wdm_set_static_potential, libwdm), Cint, (Ptr{Cvoid}, Ptr{Cdouble}), p.h, v), p.h)
+    return p
+end
+
+function _model!(p::HIPPlan, kind::Cint, params::Vector{Float64})
+    check(ccall((:wdm_set_potential_model, libwdm), Cint, (Ptr{Cvoid}, Cint, Ptr{Cdouble}, Cint),
+                p.h, kind, params, length(params)), p.h)
+    return p
+end
+set_hernquist!(p::HIPPlan; M, a, center=(0.0, 0.0, 0.0)) =
+    _model!(p, WDM_V_HERNQUIST, Float64[M, a, center...])
+set_quadrupole!(p::HIPPlan; T0::AbstractMatrix, axis=(0.0, 0.0, 1.0), period=1.0, center=(0.0, 0.0, 0.0)) =
+    _model!(p, WDM_V_QUADRUPOLE, Float64[T0[1, 1], T0[2, 2], T0[3, 3], T0[1, 2], T0[1, 3], T0[2, 3],
+                                         axis..., period, center...])
+set_kappa!(p::HIPPlan, κ::Real) = _model!(p, WDM_V_KAPPA, Float64[κ])
+
+"""
+    parallel_poisson(p::HIPPlan, ρ::Array{Float64,3}; scale=1.0) -> Φ
+
+Periodic ∇²Φ = scale·(ρ - ⟨ρ⟩) on the plan's lattice (k = 2πm/(2Xmax), k=0 → 0),
+the `Periodic()` branch of src/parallel_poisson.jl:36-68.  The reference
+returns a complex array whose imaginary part is round-off; this returns the
+real part.
+"""
+function parallel_poisson(p::HIPPlan, ρ::Array{Float64,3}; scale::Real=1.0)
+    size(ρ) == p.dims || throw(DimensionMismatch("ρ is $(size(ρ)), plan is $(p.dims)"))
+    Φ = Array{Float64,3}(undef, p.dims)
+    check(ccall((:wdm_poisson, libwdm), Cint, (Ptr{Cvoid}, Ptr{Cdouble}, Ptr{Cdouble}, Cdouble),
+                p.h, ρ, Φ, scale), p.h)
+    return Φ
+end
+
+"Φ = 4πG ∇⁻²(|ψ|² - ⟨|ψ|²⟩) of the plan's ψ (src/solver/gravity.jl:13-23), as (Φ_all, Φ_WaveDM)."
+function compute_gravitational_potential(p::HIPPlan)
+    Φ = Array{Float64,3}(undef, p.dims)
+    check(ccall((:wdm_get_phi, libwdm), Cint, (Ptr{Cvoid}, Ptr{Cdouble}), p.h, Φ), p.h)
+    return Φ, Φ
+end
+
+"""
+    apply_kick_step!(p::HIPPlan, dt_kick, t; return_phi=false)
+
+ψ ← FFT[exp(-i dt_kick (Φ[ψ] + V(t))) ψ] on the device (src/solver/KDK.jl:37-77).
+The plan keeps the spectrum (the reference's `spec`).  With `return_phi=true`
+the host Φ_all / Φ_WaveDM pair of the reference's return value is produced as
+well (one extra device-to-host copy).
+"""
+function apply_kick_step!(p::HIPPlan, dt_kick::Real, t::Real=0.0; return_phi::Bool=false)
+    Φ = return_phi ? first(compute_gravitational_potential(p)) : nothing
+    check(ccall((:wdm_kick, libwdm), Cint, (Ptr{Cvoid}, Cdouble, Cdouble), p.h, dt_kick, t), p.h)
+    return Φ, Φ, p
+end
+
+"ψ ← border .* IFFT[exp(-i k² dt/2) ψ_k] (src/solver/KDK.jl:88-101); `spec` is the plan."
+function apply_drift_step!(p::HIPPlan, dt::Real)
+    check(ccall((:wdm_drift, libwdm), Cint, (Ptr{Cvoid}, Cdouble), p.h, dt), p.h)
+    return p
+end
+
+"""
+    step!(p::HIPPlan, nsteps, dt; dt_kick=dt/2, t0=0.0, split=:reference)
+
+`nsteps` iterations of the loop body of src/simulation_main.jl:536-551
+(`split=:reference`: one kick of `dt_kick` then one drift of `dt`), or Strang
+KDK (`split=:strang`), fused on the device.
+"""
+function step!(p::HIPPlan, nsteps::Integer, dt::Real; dt_kick::Real=dt / 2, t0::Real=0.0,
+               split::Symbol=:reference)
+    mode = split === :strang ? WDM_SPLIT_STRANG : WDM_SPLIT_REFERENCE
+    check(ccall((:wdm_step, libwdm), Cint, (Ptr{Cvoid}, Cint, Cdouble, Cdouble, Cdouble, Cint),
+                p.h, nsteps, dt, dt_kick, t0, mode), p.h)
+    return p
+end
+
+"(mass, kinetic, w_self, v_ext, psi_max, phi_max) of the plan's ψ, FP64."
+function reduce_diagnostics(p::HIPPlan)
+    out = zeros(Float64, WDM_RED_COUNT)
+    check(ccall((:wdm_reduce, libwdm), Cint, (Ptr{Cvoid}, Ptr{Cdouble}, Cint), p.h, out, WDM_RED_COUNT), p.h)
+    return (mass=out[1], kinetic=out[2], w_self=out[3], v_ext=out[4], psi_max=out[5], phi_max=out[6])
+end
+
+"(dt_phase, dt_grav, dt_SI) of src/physics/poisson.jl:35-50 on the plan's ψ."
+function cfl_limits(p::HIPPlan; κ::Real=0.0)
+    out = zeros(Float64, 3)
+    check(ccall((:wdm_cfl, libwdm), Cint, (Ptr{Cvoid}, Cdouble, Ptr{Cdouble}), p.h, κ, out), p.h)
+    return (out[1], out[2], out[3])
+end
+
+end # module

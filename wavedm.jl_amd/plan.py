@@ -198,3 +198,28 @@ class Plan:
 
     def synchronize(self):
         self._chk(self.lib.wdm_synchronize(self._h))
+
+    # -- adjoint --------------------------------------------------------------
+    def adjoint_forward(self, nsteps, dt, dt_kick=None, t0=0.0, ckpt_every=20):
+        """Reference-split steps from the current psi with device checkpoints
+        every ``ckpt_every`` steps (the taped forward of the adjoint)."""
+        if dt_kick is None:
+            dt_kick = 0.5 * dt
+        self._chk(self.lib.wdm_adjoint_forward(self._h, int(nsteps), float(dt), float(dt_kick), float(t0),
+                                               int(ckpt_every)))
+
+    def adjoint_backward(self, grad_psi, dt_scale=False):
+        """Given dL/dRe(psi_N) + i dL/dIm(psi_N), return (grad_psi0, grad_params)
+        with grad_params = [dL/dM, dL/da, dL/dcx, dL/dcy, dL/dcz, dL/dsigma]
+        (dL/dsigma is NaN unless ``dt_scale``)."""
+        g = np.asarray(grad_psi)
+        if g.shape != self.shape:
+            raise ValueError(f"grad_psi must have shape {self.shape}")
+        c64 = g.dtype == np.complex64
+        gbuf = np.ascontiguousarray((g if c64 else g.astype(np.complex128)).ravel(order="F"))
+        out = np.empty(self.ncell, dtype=np.complex64 if c64 else np.complex128)
+        prm = np.zeros(A.WDM_ADJ_NPARAM, dtype=np.float64)
+        self._chk(self.lib.wdm_adjoint_backward(
+            self._h, gbuf.ctypes.data_as(C.c_void_p), A.WDM_C64 if c64 else A.WDM_C128,
+            out.ctypes.data_as(C.c_void_p), _dptr(prm), len(prm), A.WDM_ADJ_DT_SCALE if dt_scale else 0))
+        return out.reshape(self.shape, order="F"), prm
