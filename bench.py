@@ -162,6 +162,53 @@ def copy_calibration(mib, reps):
                       "gbs_read_plus_write": 2 * 4 * n / (ms * 1e-3) / 1e9}), flush=True)
 
 
+def adjoint_leg(W, steps=None):
+    """Cost of the gradient on the adjoint workload (c4: 128^3, 16 solitons,
+    Hernquist host, 200 FP32 steps, checkpoints every 20): wall time of the
+    taped forward + backward sweep against one plain forward run."""
+    cfg = W.configs.build("c4")
+    nsteps = steps or cfg["steps"]
+    with W.Plan(cfg["shape"], cfg["xmax"], "f32", hbar_over_m=cfg["hbar_over_m"],
+                four_pi_G=cfg["four_pi_G"]) as p:
+        W.configs.install(p, cfg)
+        dt, _ = W.configs.autoset_dt(p, nsteps)
+        psi0 = p.get_psi(np.complex64)
+        hq = cfg["potential"][1]
+        loss = W.adjoint.ProfileLoss(cfg["shape"], cfg["xmax"], nbins=40)
+        # target: the same run under seeded-perturbed host parameters
+        p.set_hernquist(hq["M"] * 1.05, hq["a"] * 0.95, (hq["center"][0] + 0.5, 0.2, -0.3))
+        p.set_psi(psi0)
+        p.step(nsteps, dt)
+        target = loss.profile(p.get_psi())
+        p.clear_potential_models()
+        p.set_hernquist(hq["M"], hq["a"], hq["center"])
+        # warm up both paths once, then time
+        p.set_psi(psi0)
+        p.step(2, dt)
+        W.adjoint.value_and_grad(p, psi0, 2, dt, loss, target, ckpt_every=20)
+        p.set_psi(psi0)
+        p.synchronize()
+        t0 = time.perf_counter()
+        p.step(nsteps, dt)
+        p.synchronize()
+        t_fwd = time.perf_counter() - t0
+        p.set_psi(psi0)
+        t0 = time.perf_counter()
+        p.adjoint_forward(nsteps, dt, None, 0.0, 20)
+        p.synchronize()
+        t_tape = time.perf_counter() - t0
+        L, gN = loss(p.get_psi(), target)
+        t0 = time.perf_counter()
+        g0, gp = p.adjoint_backward(gN.astype(np.complex64), dt_scale=True)
+        t_bwd = time.perf_counter() - t0
+    return {"workload": f"c4: FP32 128^3, 16 solitons + Hernquist host, {nsteps} steps, checkpoints every 20, "
+                        "40-bin radial profile loss, gradients w.r.t. psi0 and (M, a, centre, dt-scale)",
+            "forward_ms": 1e3 * t_fwd, "taped_forward_ms": 1e3 * t_tape, "backward_ms": 1e3 * t_bwd,
+            "backward_over_forward": t_bwd / t_fwd, "gradient_over_forward": (t_tape + t_bwd) / t_fwd,
+            "loss": L, "grad_params": dict(zip(W.adjoint.PARAM_NAMES, [float(v) for v in gp])),
+            "grad_psi0_norm": float(np.linalg.norm(g0))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -171,6 +218,7 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-adjoint", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -237,6 +285,9 @@ def main():
                        "hbm_gbs_step": 2 * step_bytes32 / (ms64 * 1e-3) / 1e9}
         del r64
     del r32
+    if not args.skip_adjoint:
+        print("[bench] adjoint leg (c4)", file=sys.stderr, flush=True)
+        out["adjoint"] = adjoint_leg(W)
     if not args.skip_cpu:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(W, cfg, out["config"]["dt_gyr"], args.cpu_steps)
