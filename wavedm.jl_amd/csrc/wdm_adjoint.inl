@@ -123,24 +123,6 @@ void k_adj_g(cx<T>* __restrict__ g, const cx<T>* __restrict__ psi, const double*
     }
 }
 
-// psi' = E psi (real space), for the dt-scale term
-template <class T>
-WDM_KERNEL(256)
-void k_adj_kick(cx<T>* __restrict__ out, const cx<T>* __restrict__ psi, const double* __restrict__ phi, int nx,
-                int ny, int nz, const double* __restrict__ xs, const double* __restrict__ ys,
-                const double* __restrict__ zs, VModel vm, double tau) {
-    const size_t n = size_t(nx) * ny * nz;
-    for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) {
-        int x, y, z;
-        cell_xyz(i, nx, ny, x, y, z);
-        const cx<T> pv = psi[i];
-        const double V = vm.flags ? eval_V<T>(vm, xs[x], ys[y], zs[z], unsigned(i), pv) : 0.0;
-        const cx<double> e = expmi(tau * ((phi ? phi[i] : 0.0) + V));
-        const double pr = double(pv.x), pi = double(pv.y);
-        out[i] = mk(T(e.x * pr - e.y * pi), T(e.x * pi + e.y * pr));
-    }
-}
-
 // k space: G <- conj(P) G / N and partial sums of w_k Im(conj(G_new) Psi') / N
 template <class T>
 WDM_KERNEL(256)
@@ -164,19 +146,27 @@ void k_adj_spec(cx<T>* __restrict__ G, const cx<T>* __restrict__ Psi, int nx, in
     block_sums<1>(acc, sh, partials);
 }
 
-// acc[j] += sum_b partials[NV*b + j], fixed order (one thread per component)
-WDM_KERNEL(64)
+// acc[j] += sum_b partials[nv*b + j] for j < nv: one block of 256 threads per
+// component, each thread a fixed strided subset (Neumaier), then a fixed-order
+// tree; deterministic.
+WDM_KERNEL(256)
 void k_adj_accumulate(const double* __restrict__ partials, int nb, int nv, double* __restrict__ acc) {
-    const int j = int(threadIdx.x);
-    if (j >= nv) return;
+    WDM_SHARED double sh[256];
+    const int t = int(threadIdx.x), j = int(blockIdx.x);
     double sum = 0.0, comp = 0.0;
-    for (int b = 0; b < nb; ++b) {
+    for (int b = t; b < nb; b += 256) {
         const double v = partials[nv * b + j];
-        const double t = sum + v;
-        comp += (std::fabs(sum) >= std::fabs(v)) ? (sum - t) + v : (v - t) + sum;
-        sum = t;
+        const double u = sum + v;
+        comp += (std::fabs(sum) >= std::fabs(v)) ? (sum - u) + v : (v - u) + sum;
+        sum = u;
     }
-    acc[j] += sum + comp;
+    sh[t] = sum + comp;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) sh[t] += sh[t + w];
+        __syncthreads();
+    }
+    if (t == 0) acc[j] += sh[0];
 }
 
 WDM_KERNEL(64)
@@ -193,7 +183,8 @@ struct AdjState {
     DBuf ckpt;   // (nckpt + 1) psi: psi_{kK}, k < nckpt, then psi_N
     DBuf tpsi;   // K psi of the current segment
     DBuf tphi;   // K real Phi (FP64) of the current segment
-    DBuf g, s, u, spec;  // adjoint field, real fields, k-space scratch (dt-scale)
+    DBuf tspec;  // K 3-D spectra F(psi') of the current segment (dt-scale term)
+    DBuf g, s, u;  // adjoint field and real fields
     DBuf part, acc;
     DBuf kfull[3];
 };
@@ -201,7 +192,7 @@ struct AdjState {
 void adj_release(wdm_plan* p) {
     if (!p->adj) return;
     AdjState& A = *p->adj;
-    for (DBuf* b : {&A.ckpt, &A.tpsi, &A.tphi, &A.g, &A.s, &A.u, &A.spec, &A.part, &A.acc, &A.kfull[0], &A.kfull[1],
+    for (DBuf* b : {&A.ckpt, &A.tpsi, &A.tphi, &A.tspec, &A.g, &A.s, &A.u, &A.part, &A.acc, &A.kfull[0], &A.kfull[1],
                     &A.kfull[2]})
         b->reset();
     delete p->adj;
@@ -251,7 +242,7 @@ void adj_conj_drift(wdm_plan* p, cx<T>* g, const cx<T>* Psi, double* part, doubl
     const AxisOps<T>* ox = ops_of<T>(p, 0);
     const AxisOps<T>* oy = ops_of<T>(p, 1);
     const AxisOps<T>* oz = ops_of<T>(p, 2);
-    KTabs none = {nullptr, nullptr, nullptr, 0.0};
+    KTabs none = {nullptr, nullptr, nullptr, 0.0, nullptr};
     const int nlines = p->ny * p->nz;
     ox->x_fft(-1, g, nlines, p->ax[0].tw.as<TwE>(), p->stream);
     if (p->ny > 1)
@@ -260,7 +251,7 @@ void adj_conj_drift(wdm_plan* p, cx<T>* g, const cx<T>* Psi, double* part, doubl
     if (!Psi) {
         ensure_drift(p, A.dt);
         KTabs kt = {p->ax[0].drift_conj.as<double>(), p->ax[1].drift_conj.as<double>(),
-                    p->ax[2].drift_conj.as<double>(), 1.0 / double(p->ncell)};
+                    p->ax[2].drift_conj.as<double>(), 1.0 / double(p->ncell), nullptr};
         oz->col(COL_FMI, KF_NONE, g, p->nx, p->ny, unsigned(p->nx * p->ny), unsigned(p->nx), p->ax[2].tw.as<TwE>(), kt,
                 p->stream);
     } else {
@@ -271,7 +262,7 @@ void adj_conj_drift(wdm_plan* p, cx<T>* g, const cx<T>* Psi, double* part, doubl
         WDM_LAUNCH((k_adj_spec<T>), dim3(nb), dim3(256), p->stream, g, Psi, p->nx, p->ny, p->nz,
                    A.kfull[0].as<double>(), A.kfull[1].as<double>(), A.kfull[2].as<double>(),
                    0.5 * p->hbar_over_m * A.dt, 1.0 / double(p->ncell), part);
-        WDM_LAUNCH((k_adj_accumulate), dim3(1), dim3(64), p->stream, (const double*)part, int(nb), 1, acc + 5);
+        WDM_LAUNCH((k_adj_accumulate), dim3(1), dim3(256), p->stream, (const double*)part, int(nb), 1, acc + 5);
         if (p->nz > 1)
             oz->col(COL_INV, KF_NONE, g, p->nx, p->ny, unsigned(p->nx * p->ny), unsigned(p->nx),
                     p->ax[2].tw.as<TwE>(), none, p->stream);
@@ -304,7 +295,7 @@ void adj_backward(wdm_plan* p, const void* grad, int dtype, void* grad0, double*
     double* u = p->self_gravity ? A.u.get<double>(n) : nullptr;
     double* part = A.part.get<double>(size_t(kAdjNV) * nb);
     double* acc = A.acc.get<double>(kAdjNV);
-    cx<T>* spec = want_dt ? A.spec.get<cx<T>>(n) : nullptr;
+    cx<T>* tspec = want_dt ? A.tspec.get<cx<T>>(n * size_t(A.K)) : nullptr;
     if (want_dt) {
         for (int a = 0; a < 3; ++a) {
             const int na = p->ax[a].n;
@@ -351,7 +342,13 @@ void adj_backward(wdm_plan* p, const void* grad, int dtype, void* grad0, double*
             if (tphi)
                 ox->x_c2r_real(p->rhohat.as<cx<T>>(), tphi + n * size_t(i), nlines, p->hpitch, p->ax[0].tw.as<TwE>(),
                                p->stream);
-            if (i + 1 < len) do_step<T>(p, 1, A.dt, A.dt_kick, A.t0 + double(n0 + i) * A.dt, WDM_SPLIT_REFERENCE);
+            // the step's fused z pass also tapes F(psi') for the dt-scale term;
+            // the last step of the segment only runs for that spectrum
+            if (i + 1 < len || tspec) {
+                p->zspec_out = tspec ? static_cast<void*>(tspec + n * size_t(i)) : nullptr;
+                do_step<T>(p, 1, A.dt, A.dt_kick, A.t0 + double(n0 + i) * A.dt, WDM_SPLIT_REFERENCE);
+                p->zspec_out = nullptr;
+            }
         }
         // walk the segment backwards
         for (int i = len - 1; i >= 0; --i) {
@@ -359,15 +356,11 @@ void adj_backward(wdm_plan* p, const void* grad, int dtype, void* grad0, double*
             const cx<T>* pn = tpsi + n * size_t(i);
             const double* fn = tphi ? tphi + n * size_t(i) : nullptr;
             const VModel vm = make_vmodel(p, A.t0 + double(step) * A.dt);
-            if (want_dt) {
-                WDM_LAUNCH((k_adj_kick<T>), dim3(nb), dim3(256), p->stream, spec, pn, fn, p->nx, p->ny, p->nz, xs, ys,
-                           zs, vm, tau);
-                forward3d<T>(p, spec);
-            }
-            adj_conj_drift<T>(p, g, spec, part, acc);
+            adj_conj_drift<T>(p, g, tspec ? tspec + n * size_t(i) : nullptr, part, acc);
             WDM_LAUNCH((k_adj_s<T>), dim3(nb), dim3(256), p->stream, (const cx<T>*)g, pn, fn, s, p->nx, p->ny, p->nz,
                        xs, ys, zs, vm, tau, part);
-            WDM_LAUNCH((k_adj_accumulate), dim3(1), dim3(64), p->stream, (const double*)part, int(nb), kAdjNV, acc);
+            WDM_LAUNCH((k_adj_accumulate), dim3(kAdjNV), dim3(256), p->stream, (const double*)part, int(nb), kAdjNV,
+                       acc);
             if (u) {
                 ox->x_r2c_real(s, p->rhohat.as<cx<T>>(), nlines, p->hpitch, p->ax[0].tw.as<TwE>(), p->stream);
                 poisson_chain<T>(p, p->four_pi_G);

@@ -34,7 +34,8 @@ typedef Comp PoissonPolicy;
 typedef Plain PoissonPolicy;
 #endif
 
-enum ColMode { COL_FWD = 0, COL_INV = 1, COL_FMI = 2, COL_MI = 3 };
+// COL_FMIS: COL_FMI that also stores the forward spectrum to KTabs::spec_out
+enum ColMode { COL_FWD = 0, COL_INV = 1, COL_FMI = 2, COL_MI = 3, COL_FMIS = 4 };
 // KF_PNONE: no factor, Plain arithmetic (the y passes of the Poisson chain)
 enum KFactor { KF_NONE = 0, KF_DRIFT = 1, KF_POISSON = 2, KF_PNONE = 3 };
 
@@ -48,6 +49,8 @@ struct KTabs {
     const double* oth;
     const double* line;
     double scale;
+    void* spec_out;  // COL_FMIS: the forward spectrum (before the factor) is
+                     // also stored here, same layout as data
 };
 
 template <int KF, class P, class T>
@@ -312,9 +315,18 @@ void k_col(cx<T>* __restrict__ data, int ncols, int nother, unsigned sline, unsi
         WDM_UNROLL
         for (int n1 = 0; n1 < N1; ++n1) v[n1] = WDM_LDG(ok, data, base + unsigned(n1) * step);
     }
-    constexpr int DIR = (MODE == COL_FWD || MODE == COL_FMI) ? -1 : +1;
+    constexpr bool FMI = MODE == COL_FMI || MODE == COL_FMIS;
+    constexpr int DIR = (MODE == COL_FWD || FMI) ? -1 : +1;
     fwd2<F, DIR, P>(v, X, m.j, xb, tw);
-    if constexpr (MODE == COL_FMI) {
+    if constexpr (FMI) {
+        if (MODE == COL_FMIS && ok) {
+            cx<T>* so = static_cast<cx<T>*>(kt.spec_out);
+            WDM_UNROLL
+            for (int i = 0; i < M; ++i)
+                WDM_UNROLL
+                for (int k2 = 0; k2 < N2; ++k2)
+                    stg(so, unsigned(oc) * sother + unsigned(cc) + unsigned((i * N2 + m.j) + N1 * k2) * sline, X[i][k2]);
+        }
         WDM_UNROLL
         for (int i = 0; i < M; ++i) {
             WDM_SCHED_FENCE();

@@ -104,6 +104,7 @@ struct Launch {
             case COL_FWD: col_t<COL_FWD, KF_NONE>(d, ncols, nother, sl, so, tw, kt, s); break;
             case COL_INV: col_t<COL_INV, KF_NONE>(d, ncols, nother, sl, so, tw, kt, s); break;
             case COL_FMI: col_t<COL_FMI, KF_DRIFT>(d, ncols, nother, sl, so, tw, kt, s); break;
+            case COL_FMIS: col_t<COL_FMIS, KF_DRIFT>(d, ncols, nother, sl, so, tw, kt, s); break;
             case COL_MI: col_t<COL_MI, KF_DRIFT>(d, ncols, nother, sl, so, tw, kt, s); break;
         }
     }

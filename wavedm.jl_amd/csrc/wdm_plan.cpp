@@ -142,6 +142,7 @@ struct wdm_plan {
     double last_kick_t = 0.0;
     PassTimer* timer = nullptr;
     AdjState* adj = nullptr;  // adjoint tape (wdm_adjoint_*), created on first use
+    void* zspec_out = nullptr; // when set, the fused z pass also stores the 3-D spectrum here
     std::string err;
 };
 
@@ -253,7 +254,7 @@ void poisson_chain(wdm_plan* p, double scale) {
     typedef typename TwEntry<T>::type TwE;
     cx<T>* rh = p->rhohat.as<cx<T>>();
     const int nh = p->nx / 2 + 1, hp = p->hpitch;
-    KTabs none = {nullptr, nullptr, nullptr, 0.0};
+    KTabs none = {nullptr, nullptr, nullptr, 0.0, nullptr};
     if (p->ny > 1) {
         arm(p);
         oy->col(COL_FWD, KF_PNONE, rh, nh, p->nz, unsigned(hp), unsigned(hp * p->ny), p->ax[1].tw.as<TwE>(), none,
@@ -262,7 +263,7 @@ void poisson_chain(wdm_plan* p, double scale) {
     }
     // Phi_k = -scale * rho_k / k^2, with the 1/N of the unnormalised inverse folded in
     KTabs kt = {p->ax[0].k2.as<double>(), p->ax[1].k2.as<double>(), p->ax[2].k2.as<double>(),
-                -scale / double(p->ncell)};
+                -scale / double(p->ncell), nullptr};
     arm(p);
     oz->col(COL_FMI, KF_POISSON, rh, nh, p->ny, unsigned(hp * p->ny), unsigned(hp), p->ax[2].tw.as<TwE>(), kt,
             p->stream);
@@ -399,7 +400,7 @@ void y_pass(wdm_plan* p, int mode) {
     typedef typename TwEntry<T>::type TwE;
     if (p->ny == 1) return;
     const AxisOps<T>* oy = ops_of<T>(p, 1);
-    KTabs none = {nullptr, nullptr, nullptr, 0.0};
+    KTabs none = {nullptr, nullptr, nullptr, 0.0, nullptr};
     arm(p);
     oy->col(mode, KF_NONE, p->psi.as<cx<T>>(), p->nx, p->nz, unsigned(p->nx), unsigned(p->nx * p->ny),
             p->ax[1].tw.as<TwE>(), none, p->stream);
@@ -411,11 +412,12 @@ template <class T>
 void z_pass(wdm_plan* p, int mode, double dt) {
     typedef typename TwEntry<T>::type TwE;
     const AxisOps<T>* oz = ops_of<T>(p, 2);
-    KTabs kt = {nullptr, nullptr, nullptr, 0.0};
+    KTabs kt = {nullptr, nullptr, nullptr, 0.0, nullptr};
     if (mode == COL_FMI || mode == COL_MI) {
         ensure_drift(p, dt);
         kt = {p->ax[0].drift.as<double>(), p->ax[1].drift.as<double>(), p->ax[2].drift.as<double>(),
-              1.0 / double(p->ncell)};
+              1.0 / double(p->ncell), mode == COL_FMI ? p->zspec_out : nullptr};
+        if (mode == COL_FMI && p->zspec_out) mode = COL_FMIS;
     } else if (p->nz == 1) {
         return;
     }
@@ -500,7 +502,7 @@ void forward3d(wdm_plan* p, cx<T>* buf) {
     const AxisOps<T>* ox = ops_of<T>(p, 0);
     const AxisOps<T>* oy = ops_of<T>(p, 1);
     const AxisOps<T>* oz = ops_of<T>(p, 2);
-    KTabs none = {nullptr, nullptr, nullptr, 0.0};
+    KTabs none = {nullptr, nullptr, nullptr, 0.0, nullptr};
     ox->x_fft(-1, buf, p->ny * p->nz, p->ax[0].tw.as<TwE>(), p->stream);
     if (p->ny > 1)
         oy->col(COL_FWD, KF_NONE, buf, p->nx, p->nz, unsigned(p->nx), unsigned(p->nx * p->ny),
