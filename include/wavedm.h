@@ -182,6 +182,35 @@ int wdm_adjoint_forward(wdm_plan* plan, int nsteps, double dt, double dt_kick, d
 int wdm_adjoint_backward(wdm_plan* plan, const void* grad_psi, int dtype, void* grad_psi0, double* grad_params,
                          int nparams, unsigned flags);
 
+/* Autotuner.  No counterpart in the reference.  Each axis length has up to
+   WDM_MAX_VARIANTS compiled pass-table variants (0: default factorisation and
+   workgroup size; 1: other workgroup size; 2: alternative factorisation,
+   lengths 64, 128, 192, 256).  wdm_autotune times trial_steps reference-split
+   steps of dt per variant on a copy of the plan's psi (the state is kept) and
+   selects the fastest per axis; variants_out[3] / ms_out[3] (may be NULL)
+   receive the choice and its ms per step for the axis's passes.
+   wdm_select_variants sets the three variants directly. */
+#define WDM_MAX_VARIANTS 3
+int wdm_autotune(wdm_plan* plan, int trial_steps, double dt, int* variants_out, double* ms_out);
+int wdm_select_variants(wdm_plan* plan, const int* variants);
+
+/* Batched ensemble of independent plans (uneven grids; no counterpart in the
+   reference, which steps one box at a time).  The plans stay owned by the
+   caller and must outlive the ensemble.  wdm_ensemble_step runs nsteps
+   reference-split steps of every member (member i with dt[i], dt_kick[i],
+   kick k at t0 + k dt[i]).  use_graph = 1: one step of all members is
+   captured once into a HIP graph (each member on its own stream) and
+   launched once per step; needs time-independent potentials.  use_graph = 0:
+   the same launches issued eagerly.  wdm_ensemble_autotune tunes each
+   distinct (shape, precision) once and gives the choice to every member of
+   that shape.  Errors: wdm_last_error(NULL). */
+typedef struct wdm_ensemble wdm_ensemble;
+int wdm_ensemble_create(wdm_ensemble** ens, wdm_plan* const* plans, int nplans);
+int wdm_ensemble_destroy(wdm_ensemble* ens);
+int wdm_ensemble_step(wdm_ensemble* ens, int nsteps, const double* dt, const double* dt_kick, double t0,
+                      int use_graph);
+int wdm_ensemble_autotune(wdm_ensemble* ens, int trial_steps, const double* dt);
+
 #ifdef __cplusplus
 }
 #endif

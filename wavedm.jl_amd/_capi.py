@@ -43,6 +43,7 @@ WDM_RED_PHIMAX = 5
 WDM_RED_COUNT = 6
 WDM_ADJ_DT_SCALE = 1
 WDM_ADJ_NPARAM = 6
+WDM_MAX_VARIANTS = 3
 
 # every symbol include/wavedm.h declares: name -> (restype, argtypes)
 _P = C.c_void_p
@@ -76,6 +77,12 @@ SIGNATURES = {
     "wdm_ic_solitons": (_I, [_P, _DP, _I]),
     "wdm_adjoint_forward": (_I, [_P, _I, _D, _D, _D, _I]),
     "wdm_adjoint_backward": (_I, [_P, _P, _I, _P, _DP, _I, C.c_uint]),
+    "wdm_autotune": (_I, [_P, _I, _D, C.POINTER(C.c_int), _DP]),
+    "wdm_select_variants": (_I, [_P, C.POINTER(C.c_int)]),
+    "wdm_ensemble_create": (_I, [C.POINTER(_P), C.POINTER(_P), _I]),
+    "wdm_ensemble_destroy": (_I, [_P]),
+    "wdm_ensemble_step": (_I, [_P, _I, _DP, _DP, _D, _I]),
+    "wdm_ensemble_autotune": (_I, [_P, _I, _DP]),
 }
 PASS_NAMES = ["x_kick_fwd", "y_fwd", "z_drift", "y_inv", "x_inv_density", "poisson_y_fwd", "poisson_z",
               "poisson_y_inv"]

@@ -12,6 +12,10 @@ potential, so tests, the benchmark and Julia callers build identical runs.
 * c2: 384^3, L = 20 kpc, 128 solitons within r < 5 kpc + tidal quadrupole
   V = 1/2 x^T T(t) x rotating with period 1 Gyr, eigenvalues +-(0.5-2)e3
   Gyr^-2; 1000 steps; autoset dt.
+* c3: batched ensemble of 48 boxes: 16 each of 64^3 (L = 10 kpc), 128^3
+  (L = 20 kpc) and 160^3 (L = 30 kpc), each a seeded 8-32 soliton
+  superposition (positions within L/4 of the centre) with self-gravity and no
+  external V; FP32; 200 steps; autoset dt per box.
 * c4: 128^3, L = 20 kpc, 16 solitons + the c1 Hernquist host; 200 steps.
 """
 from __future__ import annotations
@@ -57,6 +61,19 @@ def build(name: str, seed: int = 2026):
         return dict(base, name=name, shape=shape, xmax=X, precision="f32", steps=steps, dt=None,
                     psi0=None, sol_rows=IC.device_rows(params, X, hm),
                     potential=("hernquist", dict(M=1e11, a=20.0, center=(50.0, 0.0, 0.0))))
+    if name == "c3":
+        members = []
+        k = 0
+        for n, L in ((64, 10.0), (128, 20.0), (160, 30.0)):
+            for _ in range(16):
+                rng = IC.SplitMix64(seed + 1000 + k)
+                nsol = 8 + int(rng.uniform() * 25)  # 8..32
+                X = (L / 2,) * 3
+                params = IC.soliton_set(nsol, seed + 1000 + k, L / 4)
+                members.append(dict(base, name=f"c3[{k}]", shape=(n,) * 3, xmax=X, precision="f32", steps=200,
+                                    dt=None, psi0=None, sol_rows=IC.device_rows(params, X, hm), potential=None))
+                k += 1
+        return dict(base, name=name, members=members, steps=200, precision="f32")
     if name == "c2":
         shape, X = (384,) * 3, (10.0,) * 3
         params = IC.soliton_set(128, seed, 5.0)

@@ -45,7 +45,7 @@ def _deps_mtime():
     t = 0.0
     for d in (HERE, os.path.join(HERE, "emu"), os.path.join(os.path.dirname(PKG), "include")):
         for f in os.listdir(d):
-            if f.endswith((".hpp", ".h", ".cpp")):
+            if f.endswith((".hpp", ".h", ".cpp", ".inl")):
                 t = max(t, os.path.getmtime(os.path.join(d, f)))
     return t
 

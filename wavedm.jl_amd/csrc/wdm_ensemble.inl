@@ -1,0 +1,284 @@
+// Autotuner and batched ensemble (included at the end of wdm_plan.cpp; shares
+// its helpers).  No counterpart in the reference, which steps one box at a
+// time (src/simulation_main.jl:536-551).
+//
+// wdm_autotune: for every axis of a plan, time each compiled pass-table variant
+// (factorisation N1 x N2, waves per workgroup; wdm_ops.hpp) on trial steps of
+// the plan's own state, and keep the fastest per axis.
+//
+// wdm_ensemble_*: many independent plans (uneven grids, each its own stream)
+// stepped together.  One step of every member is captured once into a HIP
+// graph (fork from a root stream, each member's 8 pass kernels on its own
+// stream, join), so a step of the whole ensemble is one hipGraphLaunch and
+// the members' kernels run concurrently, filling the GPU with small grids.
+
+namespace {
+
+// per-axis cost of a pass-timing vector: x = kick + inverse/density,
+// y = psi y passes + Poisson y passes, z = drift + Poisson z
+void axis_costs(const double* ms, double cost[3]) {
+    cost[0] = ms[WDM_PASS_XKICK] + ms[WDM_PASS_XINV];
+    cost[1] = ms[WDM_PASS_YFWD] + ms[WDM_PASS_YINV] + ms[WDM_PASS_PYFWD] + ms[WDM_PASS_PYINV];
+    cost[2] = ms[WDM_PASS_ZDRIFT] + ms[WDM_PASS_PZ];
+}
+
+template <class T>
+void set_axis_ops(wdm_plan* p, int a, const AxisOps<T>* ops) {
+    ++p->gen;
+    p->ax[a].ops = ops;
+    if (a == 0) update_partials<T>(p);
+}
+
+template <class T>
+void autotune(wdm_plan* p, int trial, double dt, int* chosen, double* best_ms) {
+    if (p->kspace) fail(WDM_ERR_STATE, "wdm_autotune: psi is in k-space");
+    const size_t n = p->ncell;
+    // keep the state: trial steps run on a copy of psi
+    cx<T>* keep = p->scratch.get<cx<T>>(n);
+    hip_check(hipMemcpyAsync(keep, p->psi.as<cx<T>>(), sizeof(cx<T>) * n, hipMemcpyDeviceToDevice, p->stream),
+              "autotune save");
+    const AxisOps<T>* orig[3] = {ops_of<T>(p, 0), ops_of<T>(p, 1), ops_of<T>(p, 2)};
+    const AxisOps<T>* best[3] = {orig[0], orig[1], orig[2]};
+    double bcost[3] = {INFINITY, INFINITY, INFINITY};
+    int bvar[3] = {0, 0, 0};
+    for (int v = 0; v < kMaxVariants; ++v) {
+        const AxisOps<T>* cand[3];
+        bool any = false;
+        for (int a = 0; a < 3; ++a) {
+            cand[a] = axis_ops(p->ax[a].n, (T*)nullptr, v);
+            any = any || cand[a] != nullptr;
+        }
+        if (!any) continue;
+        for (int a = 0; a < 3; ++a) set_axis_ops<T>(p, a, cand[a] ? cand[a] : orig[a]);
+        double ms[WDM_PASS_COUNT] = {0};
+        for (int rep = 0; rep < 2; ++rep) {  // first round warms the code objects up
+            hip_check(hipMemcpyAsync(p->psi.as<cx<T>>(), keep, sizeof(cx<T>) * n, hipMemcpyDeviceToDevice, p->stream),
+                      "autotune restore");
+            p->phi_valid = false;
+            p->mass_valid = false;
+            std::fill(ms, ms + WDM_PASS_COUNT, 0.0);
+            timed_steps(p, trial, dt, 0.5 * dt, 0.0, ms, nullptr);
+        }
+        double cost[3];
+        axis_costs(ms, cost);
+        for (int a = 0; a < 3; ++a)
+            if (cand[a] && cost[a] < bcost[a]) {
+                bcost[a] = cost[a];
+                best[a] = cand[a];
+                bvar[a] = v;
+            }
+    }
+    for (int a = 0; a < 3; ++a) set_axis_ops<T>(p, a, best[a]);
+    hip_check(hipMemcpyAsync(p->psi.as<cx<T>>(), keep, sizeof(cx<T>) * n, hipMemcpyDeviceToDevice, p->stream),
+              "autotune restore");
+    hip_check(hipStreamSynchronize(p->stream), "autotune sync");
+    p->phi_valid = false;
+    p->mass_valid = false;
+    for (int a = 0; a < 3; ++a) {
+        if (chosen) chosen[a] = bvar[a];
+        if (best_ms) best_ms[a] = bcost[a] / trial;
+    }
+}
+
+template <class T>
+void select_variants(wdm_plan* p, const int* v) {
+    const AxisOps<T>* ops[3];
+    for (int a = 0; a < 3; ++a) {
+        ops[a] = axis_ops(p->ax[a].n, (T*)nullptr, v[a]);
+        if (!ops[a])
+            fail(WDM_ERR_ARG, "wdm_select_variants: axis " + std::to_string(a) + " (length " +
+                                  std::to_string(p->ax[a].n) + ") has no variant " + std::to_string(v[a]));
+    }
+    hip_check(hipStreamSynchronize(p->stream), "select sync");
+    for (int a = 0; a < 3; ++a) set_axis_ops<T>(p, a, ops[a]);
+}
+
+}  // namespace
+
+struct wdm_ensemble {
+    std::vector<wdm_plan*> m;
+    hipStream_t root = nullptr;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<double> g_dt, g_dtk;  // the step the graph holds
+    std::vector<unsigned> g_gen;      // members' configuration at capture
+};
+
+namespace {
+
+void ens_drop_graph(wdm_ensemble* e) {
+#ifdef WDM_GPU
+    if (e->exec) hipGraphExecDestroy(e->exec);
+    if (e->graph) hipGraphDestroy(e->graph);
+#endif
+    e->exec = nullptr;
+    e->graph = nullptr;
+    e->g_dt.clear();
+    e->g_dtk.clear();
+    e->g_gen.clear();
+}
+
+std::vector<unsigned> ens_gens(const wdm_ensemble* e) {
+    std::vector<unsigned> g;
+    for (const wdm_plan* p : e->m) g.push_back(p->gen);
+    return g;
+}
+
+bool time_dependent_v(const wdm_plan* p) { return (p->vflags & V_QUADRUPOLE) && p->quad[9] != 0.0; }
+
+void ens_prepare(wdm_ensemble* e, const double* dt) {
+    for (size_t i = 0; i < e->m.size(); ++i) {
+        wdm_plan* p = e->m[i];
+        if (p->kspace) fail(WDM_ERR_STATE, "wdm_ensemble_step: member " + std::to_string(i) + " is in k-space");
+        DISPATCH(p, ensure_phi, p);
+        ensure_drift(p, dt[i]);
+    }
+}
+
+// step s of every member (kick at t0 + s dt_i) on its own stream, forked from
+// and joined into root
+void ens_issue(wdm_ensemble* e, const double* dt, const double* dtk, double t0, int s) {
+    hip_check(hipEventRecord(e->fork, e->root), "fork");
+    for (size_t i = 0; i < e->m.size(); ++i) {
+        wdm_plan* p = e->m[i];
+        hip_check(hipStreamWaitEvent(p->stream, e->fork, 0), "fork wait");
+        DISPATCH(p, do_step, p, 1, dt[i], dtk[i], t0 + s * dt[i], WDM_SPLIT_REFERENCE);
+        hip_check(hipEventRecord(e->join[i], p->stream), "join");
+        hip_check(hipStreamWaitEvent(e->root, e->join[i], 0), "join wait");
+    }
+}
+
+void ens_step(wdm_ensemble* e, int nsteps, const double* dt, const double* dtk, double t0, bool use_graph) {
+    const size_t nm = e->m.size();
+    ens_prepare(e, dt);
+    for (wdm_plan* p : e->m) hip_check(hipStreamSynchronize(p->stream), "member sync");
+    if (use_graph)
+        for (wdm_plan* p : e->m)
+            if (time_dependent_v(p))
+                fail(WDM_ERR_UNSUPPORTED, "wdm_ensemble_step: the graph holds one step; a member has a "
+                                          "time-dependent potential (use eager mode)");
+#ifdef WDM_GPU
+    if (use_graph) {
+        const bool same = e->exec && e->g_dt == std::vector<double>(dt, dt + nm) &&
+                          e->g_dtk == std::vector<double>(dtk, dtk + nm) && e->g_gen == ens_gens(e);
+        if (!same) {
+            ens_drop_graph(e);
+            hip_check(hipStreamBeginCapture(e->root, hipStreamCaptureModeRelaxed), "begin capture");
+            try {
+                ens_issue(e, dt, dtk, t0, 0);
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                hipStreamEndCapture(e->root, &g);
+                if (g) hipGraphDestroy(g);
+                throw;
+            }
+            hip_check(hipStreamEndCapture(e->root, &e->graph), "end capture");
+            hip_check(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0), "graph instantiate");
+            e->g_dt.assign(dt, dt + nm);
+            e->g_dtk.assign(dtk, dtk + nm);
+            e->g_gen = ens_gens(e);
+        }
+        for (int s = 0; s < nsteps; ++s) hip_check(hipGraphLaunch(e->exec, e->root), "graph launch");
+        hip_check(hipStreamSynchronize(e->root), "ensemble sync");
+        return;
+    }
+#else
+    (void)use_graph;
+#endif
+    for (int s = 0; s < nsteps; ++s) ens_issue(e, dt, dtk, t0, s);
+    hip_check(hipStreamSynchronize(e->root), "ensemble sync");
+}
+
+template <class F>
+int ens_guarded(F&& f) {
+    return guarded(nullptr, f);  // message: wdm_last_error(NULL)
+}
+
+}  // namespace
+
+extern "C" {
+
+int wdm_autotune(wdm_plan* p, int trial_steps, double dt, int* variants_out, double* ms_out) {
+    if (!p) return fail_code(nullptr, WDM_ERR_ARG, "plan is NULL");
+    if (trial_steps < 1 || !(dt > 0) || !std::isfinite(dt))
+        return fail_code(p, WDM_ERR_ARG, "wdm_autotune: trial_steps must be >= 1 and dt > 0");
+    return guarded(p, [&] { DISPATCH(p, autotune, p, trial_steps, dt, variants_out, ms_out); });
+}
+
+int wdm_select_variants(wdm_plan* p, const int* variants) {
+    if (!p || !variants) return fail_code(p, WDM_ERR_ARG, "wdm_select_variants: NULL argument");
+    return guarded(p, [&] { DISPATCH(p, select_variants, p, variants); });
+}
+
+int wdm_ensemble_create(wdm_ensemble** out, wdm_plan* const* plans, int n) {
+    if (!out || !plans || n < 1) return fail_code(nullptr, WDM_ERR_ARG, "wdm_ensemble_create: bad arguments");
+    *out = nullptr;
+    for (int i = 0; i < n; ++i) {
+        if (!plans[i]) return fail_code(nullptr, WDM_ERR_ARG, "wdm_ensemble_create: NULL member");
+        for (int j = 0; j < i; ++j)
+            if (plans[j] == plans[i]) return fail_code(nullptr, WDM_ERR_ARG, "wdm_ensemble_create: repeated member");
+    }
+    wdm_ensemble* e = new (std::nothrow) wdm_ensemble();
+    if (!e) return fail_code(nullptr, WDM_ERR_OOM, "wdm_ensemble_create: out of host memory");
+    const int rc = guarded(nullptr, [&] {
+        e->m.assign(plans, plans + n);
+        hip_check(hipStreamCreate(&e->root), "hipStreamCreate");
+        hip_check(hipEventCreateWithFlags(&e->fork, hipEventDisableTiming), "hipEventCreate");
+        e->join.resize(n, nullptr);
+        for (int i = 0; i < n; ++i) hip_check(hipEventCreateWithFlags(&e->join[i], hipEventDisableTiming), "event");
+    });
+    if (rc != WDM_OK) {
+        wdm_ensemble_destroy(e);
+        return rc;
+    }
+    *out = e;
+    return WDM_OK;
+}
+
+int wdm_ensemble_destroy(wdm_ensemble* e) {
+    if (!e) return WDM_OK;
+    if (e->root) hipStreamSynchronize(e->root);
+    ens_drop_graph(e);
+    for (hipEvent_t ev : e->join)
+        if (ev) hipEventDestroy(ev);
+    if (e->fork) hipEventDestroy(e->fork);
+    if (e->root) hipStreamDestroy(e->root);
+    delete e;
+    return WDM_OK;
+}
+
+int wdm_ensemble_step(wdm_ensemble* e, int nsteps, const double* dt, const double* dt_kick, double t0, int use_graph) {
+    if (!e || !dt || !dt_kick || nsteps < 0) return fail_code(nullptr, WDM_ERR_ARG, "wdm_ensemble_step: bad arguments");
+    for (size_t i = 0; i < e->m.size(); ++i)
+        if (!std::isfinite(dt[i]) || !std::isfinite(dt_kick[i]))
+            return fail_code(nullptr, WDM_ERR_ARG, "wdm_ensemble_step: non-finite dt");
+    if (nsteps == 0) return WDM_OK;
+    return ens_guarded([&] { ens_step(e, nsteps, dt, dt_kick, t0, use_graph != 0); });
+}
+
+int wdm_ensemble_autotune(wdm_ensemble* e, int trial_steps, const double* dt) {
+    if (!e || !dt || trial_steps < 1) return fail_code(nullptr, WDM_ERR_ARG, "wdm_ensemble_autotune: bad arguments");
+    return ens_guarded([&] {
+        ens_drop_graph(e);
+        // members of one shape and precision share the first member's choice
+        std::vector<std::pair<std::vector<int>, std::vector<int>>> seen;
+        for (size_t i = 0; i < e->m.size(); ++i) {
+            wdm_plan* p = e->m[i];
+            std::vector<int> key = {p->nx, p->ny, p->nz, p->precision};
+            const std::vector<int>* hit = nullptr;
+            for (auto& s : seen)
+                if (s.first == key) hit = &s.second;
+            if (hit) {
+                DISPATCH(p, select_variants, p, hit->data());
+            } else {
+                int v[3];
+                DISPATCH(p, autotune, p, trial_steps, dt[i], v, nullptr);
+                seen.push_back({key, std::vector<int>(v, v + 3)});
+            }
+        }
+    });
+}
+
+}  // extern "C"

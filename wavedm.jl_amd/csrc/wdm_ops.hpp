@@ -45,6 +45,7 @@ template <class T>
 struct AxisOps {
     typedef typename TwEntry<T>::type TwE;
     int n, n1, n2, lpw;
+    int nw;  // waves per workgroup of every pass kernel of this table
     void (*col)(int mode, int kf, cx<T>* data, int ncols, int nother, unsigned sline, unsigned sother,
                 const TwE* tw, KTabs kt, hipStream_t s);
     void (*x_kick)(bool fwd, cx<T>* psi, const cx<T>* phihat, int nlines, int hpitch, const TwE* tw,
@@ -58,19 +59,19 @@ struct AxisOps {
     void (*x_fft)(int dir, cx<T>* psi, int nlines, const TwE* tw, hipStream_t s);
 };
 
-// Grid sizes (in workgroups of kNW waves).
-template <class T, class F>
+// Grid sizes (in workgroups of NW waves).
+template <int NW, class F>
 inline unsigned x_blocks(int nlines) {
     const int waves = (nlines + F::LPW - 1) / F::LPW;
-    return unsigned((waves + kNW<T> - 1) / kNW<T>);
+    return unsigned((waves + NW - 1) / NW);
 }
-template <class T, class F>
+template <int NW, class F>
 inline unsigned col_blocks(int ncols, int nother) {
     const long waves = long((ncols + F::LPW - 1) / F::LPW) * nother;
-    return unsigned((waves + kNW<T> - 1) / kNW<T>);
+    return unsigned((waves + NW - 1) / NW);
 }
 
-template <class T, class F>
+template <class T, class F, int NW = kNW<T>>
 struct Launch {
     typedef typename TwEntry<T>::type TwE;
     typedef typename std::conditional<std::is_same<T, float>::value, Comp, Plain>::type PsiP;
@@ -78,14 +79,14 @@ struct Launch {
     template <int MODE, int KF>
     static void col_t(cx<T>* d, int ncols, int nother, unsigned sl, unsigned so, const TwE* tw, KTabs kt,
                       hipStream_t s) {
-        WDM_LAUNCH((k_col<T, PsiP, F, MODE, KF, kNW<T>>), dim3(col_blocks<T, F>(ncols, nother)), dim3(kNW<T> * 64), s, d,
+        WDM_LAUNCH((k_col<T, PsiP, F, MODE, KF, NW>), dim3(col_blocks<NW, F>(ncols, nother)), dim3(NW * 64), s, d,
                    ncols, nother, sl, so, tw, kt);
     }
     typedef PoissonPolicy PoisP;
     template <int MODE, int KF>
     static void colp_t(cx<T>* d, int ncols, int nother, unsigned sl, unsigned so, const TwE* tw, KTabs kt,
                        hipStream_t s) {
-        WDM_LAUNCH((k_col<T, PoisP, F, MODE, KF, kNW<T>>), dim3(col_blocks<T, F>(ncols, nother)), dim3(kNW<T> * 64), s, d,
+        WDM_LAUNCH((k_col<T, PoisP, F, MODE, KF, NW>), dim3(col_blocks<NW, F>(ncols, nother)), dim3(NW * 64), s, d,
                    ncols, nother, sl, so, tw, kt);
     }
     // mode/kf combinations used by the plan; KF_POISSON passes run in Plain.
@@ -111,40 +112,40 @@ struct Launch {
     static void x_kick(bool fwd, cx<T>* psi, const cx<T>* ph, int nlines, int hp, const TwE* tw, const KickArgs& ka,
                        hipStream_t s) {
         if (fwd)
-            WDM_LAUNCH((k_x_kick<T, PsiP, F, true, kNW<T>>), dim3(x_blocks<T, F>(nlines)), dim3(kNW<T> * 64), s, psi, ph,
+            WDM_LAUNCH((k_x_kick<T, PsiP, F, true, NW>), dim3(x_blocks<NW, F>(nlines)), dim3(NW * 64), s, psi, ph,
                        nlines, hp, tw, ka);
         else
-            WDM_LAUNCH((k_x_kick<T, PsiP, F, false, kNW<T>>), dim3(x_blocks<T, F>(nlines)), dim3(kNW<T> * 64), s, psi, ph,
+            WDM_LAUNCH((k_x_kick<T, PsiP, F, false, NW>), dim3(x_blocks<NW, F>(nlines)), dim3(NW * 64), s, psi, ph,
                        nlines, hp, tw, ka);
     }
     static void x_inv_dens(cx<T>* psi, cx<T>* rh, int nlines, int hp, const TwE* tw, const double* border,
                            double* partials, hipStream_t s) {
-        WDM_LAUNCH((k_x_inv_dens<T, PsiP, F, kNW<T>>), dim3(x_blocks<T, F>(nlines)), dim3(kNW<T> * 64), s, psi, rh, nlines,
+        WDM_LAUNCH((k_x_inv_dens<T, PsiP, F, NW>), dim3(x_blocks<NW, F>(nlines)), dim3(NW * 64), s, psi, rh, nlines,
                    hp, tw, border, partials);
     }
     static void x_dens(const cx<T>* psi, cx<T>* rh, int nlines, int hp, const TwE* tw, double* partials,
                        hipStream_t s) {
-        WDM_LAUNCH((k_x_dens<T, F, kNW<T>>), dim3(x_blocks<T, F>(nlines)), dim3(kNW<T> * 64), s, psi, rh, nlines, hp, tw,
+        WDM_LAUNCH((k_x_dens<T, F, NW>), dim3(x_blocks<NW, F>(nlines)), dim3(NW * 64), s, psi, rh, nlines, hp, tw,
                    partials);
     }
     static void x_r2c_real(const double* src, cx<T>* rh, int nlines, int hp, const TwE* tw, hipStream_t s) {
-        WDM_LAUNCH((k_x_r2c_real<T, F, kNW<T>>), dim3(x_blocks<T, F>(nlines)), dim3(kNW<T> * 64), s, src, rh, nlines, hp,
+        WDM_LAUNCH((k_x_r2c_real<T, F, NW>), dim3(x_blocks<NW, F>(nlines)), dim3(NW * 64), s, src, rh, nlines, hp,
                    tw);
     }
     static void x_c2r_real(const cx<T>* h, double* dst, int nlines, int hp, const TwE* tw, hipStream_t s) {
-        WDM_LAUNCH((k_x_c2r_real<T, F, kNW<T>>), dim3(x_blocks<T, F>(nlines)), dim3(kNW<T> * 64), s, h, dst, nlines, hp,
+        WDM_LAUNCH((k_x_c2r_real<T, F, NW>), dim3(x_blocks<NW, F>(nlines)), dim3(NW * 64), s, h, dst, nlines, hp,
                    tw);
     }
     static void x_fft(int dir, cx<T>* psi, int nlines, const TwE* tw, hipStream_t s) {
         if (dir < 0)
-            WDM_LAUNCH((k_x_fft<T, PsiP, F, -1, kNW<T>>), dim3(x_blocks<T, F>(nlines)), dim3(kNW<T> * 64), s, psi, nlines,
+            WDM_LAUNCH((k_x_fft<T, PsiP, F, -1, NW>), dim3(x_blocks<NW, F>(nlines)), dim3(NW * 64), s, psi, nlines,
                        tw);
         else
-            WDM_LAUNCH((k_x_fft<T, PsiP, F, +1, kNW<T>>), dim3(x_blocks<T, F>(nlines)), dim3(kNW<T> * 64), s, psi, nlines,
+            WDM_LAUNCH((k_x_fft<T, PsiP, F, +1, NW>), dim3(x_blocks<NW, F>(nlines)), dim3(NW * 64), s, psi, nlines,
                        tw);
     }
     static const AxisOps<T>* table() {
-        static const AxisOps<T> ops = {F::N, F::N1, F::N2, F::LPW, &col, &x_kick, &x_inv_dens,
+        static const AxisOps<T> ops = {F::N, F::N1, F::N2, F::LPW, NW, &col, &x_kick, &x_inv_dens,
                                        &x_dens, &x_r2c_real, &x_c2r_real, &x_fft};
         return &ops;
     }
@@ -167,24 +168,68 @@ struct Launch {
     X(256, 32, 8)            \
     X(384, 48, 8)
 
-#define WDM_DECLARE_SIZE(N, A, B)                   \
-    const AxisOps<float>* axis_ops_f32_##N();       \
-    const AxisOps<double>* axis_ops_f64_##N();
+// Tuning variants per axis length (the autotuner's candidates, wdm_autotune):
+//   0  the table's factorisation, default waves per workgroup (kNW)
+//   1  the table's factorisation, kNWAlt waves per workgroup
+//   2  the alternative factorisation below, default workgroup size
+// The alternatives use N2 = 4 instead of 8: twice the values per lane for half
+// the lanes per line and twice the lines per wave (wider row segments in the
+// strided passes).
+#define WDM_FOR_EACH_ALT(X) \
+    X(64, 16, 4)            \
+    X(128, 32, 4)           \
+    X(192, 48, 4)           \
+    X(256, 64, 4)
+
+constexpr int kNWAlt = 4;
+constexpr int kMaxVariants = 3;
+
+#define WDM_DECLARE_SIZE(N, A, B)                        \
+    const AxisOps<float>* axis_ops_f32_##N(int variant); \
+    const AxisOps<double>* axis_ops_f64_##N(int variant);
 WDM_FOR_EACH_SIZE(WDM_DECLARE_SIZE)
 
-#define WDM_DEFINE_SIZE(N, A, B)                                                           \
-    const AxisOps<float>* axis_ops_f32_##N() { return Launch<float, Fac<A, B>>::table(); } \
-    const AxisOps<double>* axis_ops_f64_##N() { return Launch<double, Fac<A, B>>::table(); }
+template <int N>
+struct AltFac {
+    typedef void type;  // no alternative factorisation
+};
+#define WDM_ALT_SPEC(N, A, B)   \
+    template <>                 \
+    struct AltFac<N> {          \
+        typedef Fac<A, B> type; \
+    };
+WDM_FOR_EACH_ALT(WDM_ALT_SPEC)
 
-inline const AxisOps<float>* axis_ops(int n, float*) {
+template <class T, class F, int N>
+const AxisOps<T>* axis_variant(int variant) {
+    if (variant == 0) return Launch<T, F>::table();
+    if (variant == 1) {
+        if constexpr (kNW<T> == kNWAlt) return nullptr;
+        else return Launch<T, F, kNWAlt>::table();
+    }
+    if (variant == 2) {
+        typedef typename AltFac<N>::type A;
+        if constexpr (std::is_same<A, void>::value) return nullptr;
+        else return Launch<T, A>::table();
+    }
+    return nullptr;
+}
+
+#define WDM_DEFINE_SIZE(N, A, B)                                                                  \
+    const AxisOps<float>* axis_ops_f32_##N(int v) { return axis_variant<float, Fac<A, B>, N>(v); } \
+    const AxisOps<double>* axis_ops_f64_##N(int v) { return axis_variant<double, Fac<A, B>, N>(v); }
+
+// The pass table of axis length n in variant v (nullptr: unsupported length
+// or no such variant).
+inline const AxisOps<float>* axis_ops(int n, float*, int v = 0) {
 #define WDM_CASE_F32(N, A, B) \
-    case N: return axis_ops_f32_##N();
+    case N: return axis_ops_f32_##N(v);
     switch (n) { WDM_FOR_EACH_SIZE(WDM_CASE_F32) default: return nullptr; }
 #undef WDM_CASE_F32
 }
-inline const AxisOps<double>* axis_ops(int n, double*) {
+inline const AxisOps<double>* axis_ops(int n, double*, int v = 0) {
 #define WDM_CASE_F64(N, A, B) \
-    case N: return axis_ops_f64_##N();
+    case N: return axis_ops_f64_##N(v);
     switch (n) { WDM_FOR_EACH_SIZE(WDM_CASE_F64) default: return nullptr; }
 #undef WDM_CASE_F64
 }

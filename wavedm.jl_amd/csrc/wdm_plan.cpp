@@ -143,6 +143,7 @@ struct wdm_plan {
     PassTimer* timer = nullptr;
     AdjState* adj = nullptr;  // adjoint tape (wdm_adjoint_*), created on first use
     void* zspec_out = nullptr; // when set, the fused z pass also stores the 3-D spectrum here
+    unsigned gen = 0;          // bumped by every change a captured step would not see
     std::string err;
 };
 
@@ -853,6 +854,17 @@ void poisson_api(wdm_plan* p, const double* rho, double* phi_out, double scale) 
     hip_check(hipStreamSynchronize(p->stream), "poisson sync");
 }
 
+// partial-sum slots: one pair per launched wave of the x passes of the current
+// x-axis table (every launched wave writes its pair, including the idle tail
+// of the last group)
+template <class T>
+void update_partials(wdm_plan* p) {
+    const AxisOps<T>* ox = ops_of<T>(p, 0);
+    const size_t waves = (size_t(p->ny) * p->nz + ox->lpw - 1) / ox->lpw;
+    p->npartials = 2 * ((waves + ox->nw - 1) / ox->nw) * ox->nw;
+    p->partials.get<double>(std::max<size_t>(p->npartials, 2 * 1024));
+}
+
 template <class T>
 void init_plan(wdm_plan* p) {
     for (int a = 0; a < 3; ++a) {
@@ -864,12 +876,7 @@ void init_plan(wdm_plan* p) {
     p->psi.get<cx<T>>(p->ncell);
     p->hpitch = ((p->nx / 2 + 1) + 7) / 8 * 8;
     p->rhohat.get<cx<T>>(size_t(p->hpitch) * p->ny * p->nz);
-    // partial-sum slots: one pair per launched wave of the x passes (every
-    // launched wave writes its pair, including the idle tail of the last group)
-    const AxisOps<T>* ox = ops_of<T>(p, 0);
-    const size_t waves = (size_t(p->ny) * p->nz + ox->lpw - 1) / ox->lpw;
-    p->npartials = 2 * ((waves + kNW<T> - 1) / kNW<T>) * kNW<T>;
-    p->partials.get<double>(std::max<size_t>(p->npartials, 2 * 1024));
+    update_partials<T>(p);
     p->redout.get<double>(2);
 }
 
@@ -899,6 +906,37 @@ int guarded(wdm_plan* p, F&& f) {
         if ((p)->precision == WDM_F64) FN<double>(__VA_ARGS__); \
         else FN<float>(__VA_ARGS__);                           \
     } while (0)
+
+// nsteps reference-split steps with HIP events around every pass launch:
+// acc[pass] += summed kernel ms, *total = first-to-last event time.  Set-up
+// work (Phi of the initial psi, drift tables) runs before the timed window.
+void timed_steps(wdm_plan* p, int nsteps, double dt, double dt_kick, double t0, double* acc, double* total) {
+    PassTimer timer;
+    DISPATCH(p, ensure_phi, p);
+    ensure_drift(p, dt);
+    hip_check(hipStreamSynchronize(p->stream), "timed sync");
+    timer.reserve(size_t(WDM_PASS_COUNT) * nsteps);
+    p->timer = &timer;
+    try {
+        DISPATCH(p, do_step, p, nsteps, dt, dt_kick, t0, WDM_SPLIT_REFERENCE);
+    } catch (...) {
+        p->timer = nullptr;
+        throw;
+    }
+    p->timer = nullptr;
+    hip_check(hipStreamSynchronize(p->stream), "timed sync");
+    for (size_t i = 0; i < timer.used; ++i) {
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, timer.ev[2 * i], timer.ev[2 * i + 1]), "hipEventElapsedTime");
+        if (timer.id[i] >= 0 && timer.id[i] < WDM_PASS_COUNT) acc[timer.id[i]] += ms;
+    }
+    if (total) {
+        float ms = 0.f;
+        if (timer.used > 0)
+            hip_check(hipEventElapsedTime(&ms, timer.ev[0], timer.ev[2 * timer.used - 1]), "hipEventElapsedTime");
+        *total = ms;
+    }
+}
 
 }  // namespace
 
@@ -979,6 +1017,7 @@ int wdm_plan_destroy(wdm_plan* p) {
 }
 
 int wdm_set_coeffs(wdm_plan* p, double hbar_over_m, double four_pi_G) {
+    if (p) ++p->gen;
     if (!p) return fail_code(nullptr, WDM_ERR_ARG, "plan is NULL");
     if (!(hbar_over_m > 0) || !std::isfinite(four_pi_G))
         return fail_code(p, WDM_ERR_ARG, "wdm_set_coeffs: hbar_over_m must be > 0 and four_pi_G finite");
@@ -989,6 +1028,7 @@ int wdm_set_coeffs(wdm_plan* p, double hbar_over_m, double four_pi_G) {
 }
 
 int wdm_set_self_gravity(wdm_plan* p, int on) {
+    if (p) ++p->gen;
     if (!p) return fail_code(nullptr, WDM_ERR_ARG, "plan is NULL");
     p->self_gravity = on != 0;
     p->phi_valid = false;
@@ -1020,6 +1060,7 @@ int wdm_get_psi_device(wdm_plan* p, void* dev, int dtype) {
 }
 
 int wdm_set_border(wdm_plan* p, const double* border, int rank) {
+    if (p) ++p->gen;
     if (!p) return fail_code(nullptr, WDM_ERR_ARG, "plan is NULL");
     if (!border) {
         p->border_on = false;
@@ -1034,6 +1075,7 @@ int wdm_set_border(wdm_plan* p, const double* border, int rank) {
 }
 
 int wdm_set_static_potential(wdm_plan* p, const double* V) {
+    if (p) ++p->gen;
     if (!p) return fail_code(nullptr, WDM_ERR_ARG, "plan is NULL");
     if (!V) {
         p->vflags &= ~V_STATIC;
@@ -1048,6 +1090,7 @@ int wdm_set_static_potential(wdm_plan* p, const double* V) {
 }
 
 int wdm_set_potential_model(wdm_plan* p, int kind, const double* params, int nparams) {
+    if (p) ++p->gen;
     if (!p) return fail_code(nullptr, WDM_ERR_ARG, "plan is NULL");
     if (!params) return fail_code(p, WDM_ERR_ARG, "wdm_set_potential_model: params is NULL");
     switch (kind) {
@@ -1072,6 +1115,7 @@ int wdm_set_potential_model(wdm_plan* p, int kind, const double* params, int npa
 }
 
 int wdm_clear_potential_models(wdm_plan* p) {
+    if (p) ++p->gen;
     if (!p) return fail_code(nullptr, WDM_ERR_ARG, "plan is NULL");
     p->vflags &= V_STATIC;
     return WDM_OK;
@@ -1153,34 +1197,12 @@ int wdm_step_timed(wdm_plan* p, int nsteps, double dt, double dt_kick, double t0
     if (nsteps < 0 || !std::isfinite(dt) || !std::isfinite(dt_kick) || !std::isfinite(t0))
         return fail_code(p, WDM_ERR_ARG, "wdm_step_timed: bad argument");
     if (p->kspace) return fail_code(p, WDM_ERR_STATE, "wdm_step_timed: psi is in k-space");
-    PassTimer timer;
-    const int rc = guarded(p, [&] {
-        // set-up work (Phi of the initial psi, drift tables) stays outside the timed window
-        DISPATCH(p, ensure_phi, p);
-        ensure_drift(p, dt);
-        hip_check(hipStreamSynchronize(p->stream), "timed sync");
-        timer.reserve(size_t(WDM_PASS_COUNT) * nsteps);
-        p->timer = &timer;
-        DISPATCH(p, do_step, p, nsteps, dt, dt_kick, t0, WDM_SPLIT_REFERENCE);
-        p->timer = nullptr;
-        hip_check(hipStreamSynchronize(p->stream), "timed sync");
-        double acc[WDM_PASS_COUNT] = {0};
-        for (size_t i = 0; i < timer.used; ++i) {
-            float ms = 0.f;
-            hip_check(hipEventElapsedTime(&ms, timer.ev[2 * i], timer.ev[2 * i + 1]), "hipEventElapsedTime");
-            if (timer.id[i] >= 0 && timer.id[i] < WDM_PASS_COUNT) acc[timer.id[i]] += ms;
-        }
+    return guarded(p, [&] {
+        double acc[WDM_PASS_COUNT] = {0}, tot = 0.0;
+        timed_steps(p, nsteps, dt, dt_kick, t0, acc, &tot);
         for (int i = 0; i < npass && i < WDM_PASS_COUNT; ++i) pass_ms[i] = acc[i];
-        if (total_ms) {
-            float ms = 0.f;
-            if (timer.used > 0)
-                hip_check(hipEventElapsedTime(&ms, timer.ev[0], timer.ev[2 * timer.used - 1]),
-                          "hipEventElapsedTime");
-            *total_ms = ms;
-        }
+        if (total_ms) *total_ms = tot;
     });
-    p->timer = nullptr;
-    return rc;
 }
 
 int wdm_ic_solitons(wdm_plan* p, const double* params, int nsol) {
@@ -1196,3 +1218,4 @@ int wdm_synchronize(wdm_plan* p) {
 }  // extern "C"
 
 #include "wdm_adjoint.inl"
+#include "wdm_ensemble.inl"

@@ -199,6 +199,20 @@ class Plan:
     def synchronize(self):
         self._chk(self.lib.wdm_synchronize(self._h))
 
+    # -- tuning ---------------------------------------------------------------
+    def autotune(self, trial_steps, dt):
+        """Time every compiled pass-table variant per axis on ``trial_steps``
+        steps of a copy of the state and keep the fastest; returns
+        (variants[3], ms per step of each axis's passes[3])."""
+        v = (C.c_int * 3)()
+        ms = np.zeros(3, dtype=np.float64)
+        self._chk(self.lib.wdm_autotune(self._h, int(trial_steps), float(dt), v, _dptr(ms)))
+        return [int(x) for x in v], ms
+
+    def select_variants(self, variants):
+        v = (C.c_int * 3)(*[int(x) for x in variants])
+        self._chk(self.lib.wdm_select_variants(self._h, v))
+
     # -- adjoint --------------------------------------------------------------
     def adjoint_forward(self, nsteps, dt, dt_kick=None, t0=0.0, ckpt_every=20):
         """Reference-split steps from the current psi with device checkpoints
