@@ -209,6 +209,48 @@ def adjoint_leg(W, steps=None):
             "grad_psi0_norm": float(np.linalg.norm(g0))}
 
 
+def ensemble_leg(W, steps=None):
+    """The batched workload (c3: 48 boxes, 16 each of 64^3, 128^3, 160^3, FP32,
+    self-gravity): autotune per shape, then time the steps as one HIP-graph
+    launch per step and as eager per-member launches."""
+    cfg = W.configs.build("c3")
+    nsteps = steps or cfg["steps"]
+    plans = []
+    try:
+        dts = []
+        for m in cfg["members"]:
+            p = W.Plan(m["shape"], m["xmax"], "f32", hbar_over_m=m["hbar_over_m"], four_pi_G=m["four_pi_G"])
+            plans.append(p)
+            W.configs.install(p, m)
+            dts.append(W.configs.autoset_dt(p, nsteps)[0])
+        cells = sum(p.ncell for p in plans)
+        psi0 = [p.get_psi(np.complex64) for p in plans]
+        out = {"workload": f"c3: 48 FP32 boxes (16 x 64^3, 16 x 128^3, 16 x 160^3), self-gravity, "
+                           f"{nsteps} steps, autoset dt per box", "cells": cells}
+        with W.Ensemble(plans) as e:
+            t0 = time.perf_counter()
+            e.autotune(2, dts)
+            out["autotune_s"] = time.perf_counter() - t0
+            for mode in ("graph", "eager"):
+                for p, s0 in zip(plans, psi0):
+                    p.set_psi(s0)
+                e.step(2, dts, graph=mode == "graph")   # capture / warm-up
+                for p, s0 in zip(plans, psi0):
+                    p.set_psi(s0)
+                for p in plans:
+                    p.synchronize()
+                t0 = time.perf_counter()
+                e.step(nsteps, dts, graph=mode == "graph")
+                wall = time.perf_counter() - t0
+                out[mode] = {"ms_per_step": 1e3 * wall / nsteps, "cell_steps_per_s": cells * nsteps / wall,
+                             "hbm_frac_112B": 112.0 * cells * nsteps / wall / (HBM_PEAK_GBS * 1e9)}
+        out["mass_finite"] = bool(all(np.isfinite(p.reduce(1)["mass"]) for p in plans))
+        return out
+    finally:
+        for p in plans:
+            p.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -219,6 +261,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-adjoint", action="store_true")
+    ap.add_argument("--skip-ensemble", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -288,6 +331,9 @@ def main():
     if not args.skip_adjoint:
         print("[bench] adjoint leg (c4)", file=sys.stderr, flush=True)
         out["adjoint"] = adjoint_leg(W)
+    if not args.skip_ensemble:
+        print("[bench] ensemble leg (c3)", file=sys.stderr, flush=True)
+        out["ensemble"] = ensemble_leg(W)
     if not args.skip_cpu:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(W, cfg, out["config"]["dt_gyr"], args.cpu_steps)
