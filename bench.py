@@ -64,6 +64,17 @@ def pass_bytes(shape, elem_bytes):
     }
 
 
+def measured_traffic(shape, kernel):
+    """HBM bytes per launch of one FP32 pass from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (profiles/traffic_384.json, scaled by the
+    copy-kernel calibration), or None when there is none for this grid."""
+    path = os.path.join(ROOT, "profiles", "traffic_384.json")
+    if tuple(shape) != (384, 384, 384) or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)["bytes_per_launch"].get("FP32 " + kernel)
+
+
 def energy(red):
     return red["kinetic"] + red["w_self"] + red["v_ext"]
 
@@ -307,7 +318,8 @@ def main():
                    "grid": list(cfg["shape"]), "dt_gyr": r32["dt"], "cfl_limits": r32["cfl"]},
         "hbm_gbs_step": step_bytes32 / (ms32 * 1e-3) / 1e9,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(cfg["shape"], dom),
+                     "algorithmic_bytes": bytes32[dom],
                      "step_frac": step_bytes32 / (ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "passes": passes,
     }
