@@ -248,6 +248,17 @@ WDM_FI cx<float> scale_dyn(cx<float> a, double f) {
 }
 
 // ------------------------------------------------------------------ wave helpers
+// Copies a small table (twiddles, k-space factors) into LDS once per
+// workgroup and returns the LDS copy.  The line FFTs index these tables per
+// lane; from LDS the lookups cost a short LDS round trip instead of an L2 one
+// each, which the compiler would otherwise wait for one by one.
+template <int N, class E>
+WDM_FI const E* stage_table(E* dst, const E* __restrict__ src) {
+    for (int i = int(threadIdx.x); i < N; i += int(blockDim.x)) dst[i] = src[i];
+    __syncthreads();
+    return dst;
+}
+
 struct LaneMap {
     int lane, wv, j, lw, w;
 };
@@ -281,16 +292,237 @@ WDM_FI void wave_sum_max(cx<T>* wbuf, int lane, double vsum, double vmax, double
     WDM_WAVE_SYNC();
 }
 
+// ------------------------------------------------------------------ LDS-staged column pass
+// Same work as k_col.  Each wave owns a staging buffer in LDS for one tile
+// (LPW columns x N rows) and walks `tpw` tiles (stride: all waves of the
+// grid).  The tile is fetched with direct global->LDS copies (no registers),
+// read into registers, and the next tile's copies are issued at once, so the
+// HBM reads of tile k+1 overlap the transforms and stores of tile k within the
+// same wave.  A copy instruction moves 64 lanes x 16 bytes: RPI rows of LPW
+// columns, LDS layout [row][column].
+template <class F, class T>
+struct Stage {
+    static constexpr int E = 16 / int(sizeof(cx<T>));   // elements per lane copy
+    static constexpr int LPR = F::LPW / E;               // lanes per row
+    static constexpr int RPI = kWave / LPR;              // rows per copy instruction
+    static constexpr int NI = F::N / RPI;                // copy instructions per tile
+    static constexpr int ELEMS = F::N * F::LPW;          // tile elements
+    static constexpr bool OK = (F::LPW % E == 0) && (kWave % LPR == 0) && (F::N % RPI == 0) && F::N2 > 1;
+};
+
+template <class T, class P, class F, int MODE, int KF, int NW>
+WDM_KERNEL(NW * 64)
+void k_cols(cx<T>* __restrict__ data, int ncols, int nother, unsigned sline, unsigned sother,
+            const typename TwEntry<T>::type* __restrict__ tw, KTabs kt, int tpw) {
+    typedef Stage<F, T> S;
+    static_assert(S::OK, "tile shape not stageable");
+    static_assert(MODE != COL_FMIS, "the spectrum side output uses k_col");
+    constexpr int N1 = F::N1, N2 = F::N2, M = F::M, LPW = F::LPW;
+    constexpr bool FMI = MODE == COL_FMI;
+    constexpr int DIR = (MODE == COL_FWD || FMI) ? -1 : +1;
+    static_assert(sizeof(cx<T>) * NW * (F::XBUF + S::ELEMS) <= 160 * 1024, "LDS of one CU exceeded");
+    WDM_SHARED cx<T> xbuf[NW][F::XBUF];
+    WDM_SHARED typename TwEntry<T>::type tw_s[F::N];
+    tw = stage_table<F::N>(tw_s, tw);
+    WDM_SHARED cx<T> stage[NW][S::ELEMS];
+    const LaneMap m = lane_map<F, NW>();
+    const int lane = m.lane;
+    const int ntiles = (ncols + LPW - 1) / LPW;
+    const int total = ntiles * nother;
+    const int stride = int(gridDim.x) * NW;
+    const unsigned step = unsigned(N2) * sline;
+    cx<T>* xb = &xbuf[m.wv][m.lw * F::XS];
+    cx<T>* sb = stage[m.wv];
+    // this lane's part of every copy instruction: row offset and column
+    const int crow = lane / S::LPR, ccol = (lane % S::LPR) * S::E;
+    auto fetch = [&](int t) {  // wave-uniform t < total
+        const int o = t / ntiles, c0 = (t % ntiles) * LPW;
+        const bool cok = c0 + ccol < ncols;  // a lane's E columns are all in or all out (ncols % E == 0)
+        const unsigned base = unsigned(o) * sother + unsigned(c0 + ccol) + unsigned(crow) * sline;
+        if (cok) {
+            WDM_UNROLL
+            for (int r = 0; r < S::NI; ++r)
+                lds_dma16(data + (base + unsigned(r * S::RPI) * sline), sb + r * (kWave * S::E));
+        }
+    };
+    int t = m.w;
+    if (t >= total) return;
+    fetch(t);
+    for (int k = 0; k < tpw; ++k) {
+        const int o = t / ntiles, c = (t % ntiles) * LPW + m.lw;
+        const bool ok = c < ncols;
+        const int cc = ok ? c : 0;
+        const int tn = t + stride;
+        const bool more = (k + 1 < tpw) && (tn < total);
+        WDM_WAIT_VMEM();  // the tile's copies have landed
+        WDM_WAVE_SYNC();
+        cx<T> v[N1];
+        WDM_UNROLL
+        for (int n1 = 0; n1 < N1; ++n1) v[n1] = sb[(N2 * n1 + m.j) * LPW + m.lw];
+        WDM_WAIT_LDS();   // every lane has read the buffer before it is refilled
+        WDM_WAVE_SYNC();
+        if (more) fetch(tn);
+        if constexpr (MODE == COL_MI) {
+            WDM_UNROLL
+            for (int n1 = 0; n1 < N1; ++n1) v[n1] = apply_kf<KF, P, T>(v[n1], kt, cc, o, N2 * n1 + m.j);
+        }
+        cx<T> X[M][N2];
+        fwd2<F, DIR, P>(v, X, m.j, xb, tw);
+        if constexpr (FMI) {
+            WDM_UNROLL
+            for (int i = 0; i < M; ++i) {
+                WDM_SCHED_FENCE();
+                WDM_UNROLL
+                for (int k2 = 0; k2 < N2; ++k2)
+                    X[i][k2] = apply_kf<KF, P, T>(X[i][k2], kt, cc, o, (i * N2 + m.j) + N1 * k2);
+            }
+            inv2<F, +1, P>(X, v, m.j, xb, tw);
+            if (ok) {
+                const unsigned sbase = unsigned(o) * sother + unsigned(cc) + unsigned(m.j) * sline;
+                WDM_UNROLL
+                for (int kq = 0; kq < N1; ++kq) stg(data, sbase + unsigned(kq) * step, v[kq]);
+            }
+        } else {
+            if (ok) {
+                WDM_UNROLL
+                for (int i = 0; i < M; ++i)
+                    WDM_UNROLL
+                    for (int k2 = 0; k2 < N2; ++k2)
+                        stg(data, unsigned(o) * sother + unsigned(cc) + unsigned((i * N2 + m.j) + N1 * k2) * sline,
+                            X[i][k2]);
+            }
+        }
+        if (!more) break;
+        t = tn;
+    }
+}
+
+// ------------------------------------------------------------------ pipelined column pass
+// Same work as k_col, but each wave walks TPW tiles (stride: all waves of the
+// grid) and issues the loads of tile k+1 before the transforms of tile k, so
+// HBM reads overlap the FFT arithmetic of the same wave instead of relying on
+// other resident waves.  Tiles past the end are idle (no loads, no stores).
+template <class T, class P, class F, int MODE, int KF, int NW, int TPW>
+WDM_KERNEL(NW * 64)
+void k_colp(cx<T>* __restrict__ data, int ncols, int nother, unsigned sline, unsigned sother,
+            const typename TwEntry<T>::type* __restrict__ tw, KTabs kt) {
+    constexpr int N1 = F::N1, N2 = F::N2, M = F::M;
+    static_assert(MODE != COL_FMIS, "the spectrum side output uses k_col");
+    constexpr bool FMI = MODE == COL_FMI;
+    constexpr int DIR = (MODE == COL_FWD || FMI) ? -1 : +1;
+    WDM_SHARED cx<T> xbuf[NW][F::XBUF];
+    WDM_SHARED typename TwEntry<T>::type tw_s[F::N];
+    tw = stage_table<F::N>(tw_s, tw);
+    const LaneMap m = lane_map<F, NW>();
+    const int ntiles = (ncols + F::LPW - 1) / F::LPW;
+    const int total = ntiles * nother;
+    const int stride = int(gridDim.x) * NW;
+    const unsigned step = unsigned(N2) * sline;
+    cx<T>* xb = &xbuf[m.wv][m.lw * F::XS];
+    const cx<T> zero = mk(T(0), T(0));
+    // tile t -> (ok, other, column)
+    auto geo = [&](int t, bool& ok, int& oc, int& cc) {
+        const int o = t / ntiles, c = (t % ntiles) * F::LPW + m.lw;
+        ok = (t < total) && (c < ncols);
+        oc = ok ? o : 0;
+        cc = ok ? c : 0;
+    };
+    auto load = [&](cx<T>* v, bool ok, int oc, int cc) {
+        const unsigned base = unsigned(oc) * sother + unsigned(cc) + unsigned(m.j) * sline;
+        WDM_UNROLL
+        for (int n1 = 0; n1 < N1; ++n1) v[n1] = WDM_LDG(ok, data, base + unsigned(n1) * step);
+    };
+    int t = m.w;
+    bool ok;
+    int oc, cc;
+    geo(t, ok, oc, cc);
+    cx<T> v[N1];
+    load(v, ok, oc, cc);
+    for (int k = 0; k < TPW; ++k) {
+        if (t >= total) break;  // wave-uniform
+        const int tn = t + stride;
+        const bool more = (k + 1 < TPW) && (tn < total);
+        bool nok = false;
+        int noc = 0, ncc = 0;
+        cx<T> nv[N1];
+        if (more) {
+            geo(tn, nok, noc, ncc);
+            load(nv, nok, noc, ncc);
+        }
+        WDM_SCHED_FENCE();
+        if constexpr (MODE == COL_MI) {
+            WDM_UNROLL
+            for (int n1 = 0; n1 < N1; ++n1) v[n1] = apply_kf<KF, P, T>(v[n1], kt, cc, oc, N2 * n1 + m.j);
+        }
+        cx<T> X[M][N2];
+        fwd2<F, DIR, P>(v, X, m.j, xb, tw);
+        if constexpr (FMI) {
+            WDM_UNROLL
+            for (int i = 0; i < M; ++i) {
+                WDM_SCHED_FENCE();
+                WDM_UNROLL
+                for (int k2 = 0; k2 < N2; ++k2)
+                    X[i][k2] = apply_kf<KF, P, T>(X[i][k2], kt, cc, oc, (i * N2 + m.j) + N1 * k2);
+            }
+            inv2<F, +1, P>(X, v, m.j, xb, tw);
+            if (ok) {
+                const unsigned sbase = unsigned(oc) * sother + unsigned(cc) + unsigned(m.j) * sline;
+                WDM_UNROLL
+                for (int kq = 0; kq < N1; ++kq) stg(data, sbase + unsigned(kq) * step, v[kq]);
+            }
+        } else {
+            if (ok) {
+                WDM_UNROLL
+                for (int i = 0; i < M; ++i)
+                    WDM_UNROLL
+                    for (int k2 = 0; k2 < N2; ++k2)
+                        stg(data, unsigned(oc) * sother + unsigned(cc) + unsigned((i * N2 + m.j) + N1 * k2) * sline,
+                            X[i][k2]);
+            }
+        }
+        if (!more) break;
+        WDM_UNROLL
+        for (int n1 = 0; n1 < N1; ++n1) v[n1] = nv[n1];
+        ok = nok;
+        oc = noc;
+        cc = ncc;
+        t = tn;
+    }
+}
+
 // ------------------------------------------------------------------ column pass (y or z)
 // Lines along an axis with element stride `sline`; columns are the contiguous
 // x index c in [0, ncols); `nother` positions of the remaining axis at stride
 // `sother`.  Each wave owns LPW adjacent columns at one `other` position.
+// minimum waves per SIMD asked of the compiler for a pass kernel: the fused
+// FFT * factor * IFFT passes in FP32 sit just above the 168-register line of
+// three waves per SIMD
+#ifndef WDM_MINW_FMI_F32
+#define WDM_MINW_FMI_F32 1
+#endif
+#ifndef WDM_MINW_XINV_F32
+#define WDM_MINW_XINV_F32 1
+#endif
+template <class T, int MODE>
+constexpr int col_min_waves() {
+    return (std::is_same<T, float>::value && (MODE == COL_FMI)) ? WDM_MINW_FMI_F32 : 1;
+}
+
 template <class T, class P, class F, int MODE, int KF, int NW>
-WDM_KERNEL(NW * 64)
+WDM_KERNEL_W(NW * 64, (col_min_waves<T, MODE>()))
 void k_col(cx<T>* __restrict__ data, int ncols, int nother, unsigned sline, unsigned sother,
            const typename TwEntry<T>::type* __restrict__ tw, KTabs kt) {
     constexpr int N1 = F::N1, N2 = F::N2, M = F::M;
     WDM_SHARED cx<T> xbuf[NW][F::XBUF];
+    WDM_SHARED typename TwEntry<T>::type tw_s[F::N];
+    // k-space factor of the line axis (complex drift factors or real k^2)
+    constexpr int NL = KF == KF_DRIFT ? 2 * F::N : (KF == KF_POISSON ? F::N : 1);
+    WDM_SHARED double line_s[NL];
+    if constexpr (KF == KF_DRIFT || KF == KF_POISSON) {
+        for (int i = int(threadIdx.x); i < NL; i += int(blockDim.x)) line_s[i] = kt.line[i];
+        kt.line = line_s;
+    }
+    tw = stage_table<F::N>(tw_s, tw);
     const LaneMap m = lane_map<F, NW>();
     const int ntiles = (ncols + F::LPW - 1) / F::LPW;
     const int o = m.w / ntiles, tile = m.w % ntiles;
@@ -375,6 +607,8 @@ void k_x_kick(cx<T>* __restrict__ psi, const cx<T>* __restrict__ phihat, int nli
               const typename TwEntry<T>::type* __restrict__ tw, KickArgs ka) {
     constexpr int N1 = F::N1, N2 = F::N2, M = F::M, N = F::N;
     WDM_SHARED cx<T> xbuf[NW][F::XBUF];
+    WDM_SHARED typename TwEntry<T>::type tw_s[F::N];
+    tw = stage_table<F::N>(tw_s, tw);
     const LaneMap m = lane_map<F, NW>();
     const int l = m.w * F::LPW + m.lw;
     const bool ok = l < nlines;
@@ -486,12 +720,14 @@ WDM_FI void density_r2c(const cx<T>* v, bool ok, const LaneMap& m, cx<T>* xb, cx
 // then rho = |psi|^2 -> R2C_x -> rho_hat.  `border` is a length-Nx FP64
 // vector or null (src/solver/KDK.jl:95-96; broadcast along x, Q4).
 template <class T, class P, class F, int NW>
-WDM_KERNEL(NW * 64)
+WDM_KERNEL_W(NW * 64, (std::is_same<T, float>::value ? WDM_MINW_XINV_F32 : 1))
 void k_x_inv_dens(cx<T>* __restrict__ psi, cx<T>* __restrict__ rhohat, int nlines, int hpitch,
                   const typename TwEntry<T>::type* __restrict__ tw, const double* __restrict__ border,
                   double* __restrict__ partials) {
     constexpr int N1 = F::N1, N2 = F::N2, M = F::M, N = F::N;
     WDM_SHARED cx<T> xbuf[NW][F::XBUF];
+    WDM_SHARED typename TwEntry<T>::type tw_s[F::N];
+    tw = stage_table<F::N>(tw_s, tw);
     const LaneMap m = lane_map<F, NW>();
     const int l = m.w * F::LPW + m.lw;
     const bool ok = l < nlines;
@@ -527,6 +763,8 @@ void k_x_dens(const cx<T>* __restrict__ psi, cx<T>* __restrict__ rhohat, int nli
               const typename TwEntry<T>::type* __restrict__ tw, double* __restrict__ partials) {
     constexpr int N1 = F::N1, N2 = F::N2, N = F::N;
     WDM_SHARED cx<T> xbuf[NW][F::XBUF];
+    WDM_SHARED typename TwEntry<T>::type tw_s[F::N];
+    tw = stage_table<F::N>(tw_s, tw);
     const LaneMap m = lane_map<F, NW>();
     const int l = m.w * F::LPW + m.lw;
     const bool ok = l < nlines;
@@ -547,6 +785,8 @@ void k_x_r2c_real(const double* __restrict__ src, cx<T>* __restrict__ rhohat, in
                   const typename TwEntry<T>::type* __restrict__ tw) {
     constexpr int N1 = F::N1, N2 = F::N2, M = F::M, N = F::N;
     WDM_SHARED cx<T> xbuf[NW][F::XBUF];
+    WDM_SHARED typename TwEntry<T>::type tw_s[F::N];
+    tw = stage_table<F::N>(tw_s, tw);
     const LaneMap m = lane_map<F, NW>();
     const int l = m.w * F::LPW + m.lw;
     const bool ok = l < nlines;
@@ -577,6 +817,8 @@ void k_x_c2r_real(const cx<T>* __restrict__ hsrc, double* __restrict__ dst, int 
                   const typename TwEntry<T>::type* __restrict__ tw) {
     constexpr int N1 = F::N1, N2 = F::N2, M = F::M, N = F::N;
     WDM_SHARED cx<T> xbuf[NW][F::XBUF];
+    WDM_SHARED typename TwEntry<T>::type tw_s[F::N];
+    tw = stage_table<F::N>(tw_s, tw);
     const LaneMap m = lane_map<F, NW>();
     const int l = m.w * F::LPW + m.lw;
     const bool ok = l < nlines;
@@ -614,6 +856,8 @@ WDM_KERNEL(NW * 64)
 void k_x_fft(cx<T>* __restrict__ psi, int nlines, const typename TwEntry<T>::type* __restrict__ tw) {
     constexpr int N1 = F::N1, N2 = F::N2, M = F::M, N = F::N;
     WDM_SHARED cx<T> xbuf[NW][F::XBUF];
+    WDM_SHARED typename TwEntry<T>::type tw_s[F::N];
+    tw = stage_table<F::N>(tw_s, tw);
     const LaneMap m = lane_map<F, NW>();
     const int l = m.w * F::LPW + m.lw;
     const bool ok = l < nlines;

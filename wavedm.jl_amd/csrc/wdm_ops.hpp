@@ -40,6 +40,35 @@ namespace wdm {
 #endif
 template <class T>
 constexpr int kNW = sizeof(T) == 4 ? WDM_NW_F32 : WDM_NW_F64;
+// tiles per wave of the column passes (1: one tile per wave, no prefetch)
+#ifndef WDM_COL_TPW
+#define WDM_COL_TPW 1
+#endif
+// workgroup size of the fused z passes when they ask for more than one wave
+// per SIMD (a multiple of the waves per SIMD wastes no slots)
+#ifndef WDM_NW_FMI
+#define WDM_NW_FMI 4
+#endif
+// LDS-staged persistent column passes (k_cols) where the tile shape allows
+#ifndef WDM_COL_STAGED
+#define WDM_COL_STAGED 0
+#endif
+
+// compute units of the current device (cached for the process)
+inline int device_cus() {
+#ifdef WDM_GPU
+    static int cus = [] {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
+            v = 256;
+        return v;
+    }();
+    return cus;
+#else
+    return 4;
+#endif
+}
 
 template <class T>
 struct AxisOps {
@@ -76,18 +105,58 @@ struct Launch {
     typedef typename TwEntry<T>::type TwE;
     typedef typename std::conditional<std::is_same<T, float>::value, Comp, Plain>::type PsiP;
 
+    // staged variant: one workgroup per CU walking tpw tiles per wave
+    template <class PP, int MODE, int KF>
+    static bool col_staged(cx<T>* d, int ncols, int nother, unsigned sl, unsigned so, const TwE* tw, KTabs kt,
+                           hipStream_t s) {
+        typedef Stage<F, T> S;
+        // 4 waves per workgroup: 4 staging buffers + transpose buffers fit
+        // the 160 KB of LDS of one CU
+        constexpr int SNW = 4;
+        if constexpr (!WDM_COL_STAGED || !S::OK || MODE == COL_FMIS || !std::is_same<T, float>::value) {
+            return false;
+        } else {
+            // a lane copies E adjacent columns: the last one read must stay
+            // inside the row (pitch = elements between rows of columns)
+            const unsigned pitch = sl < so ? sl : so;
+            if (unsigned((ncols + S::E - 1) / S::E * S::E) > pitch) return false;
+            const long tiles = long((ncols + F::LPW - 1) / F::LPW) * nother;
+            const long waves = long(device_cus()) * SNW;
+            const int tpw = int((tiles + waves - 1) / waves);
+            const unsigned blocks = unsigned((tiles + long(SNW) * tpw - 1) / (long(SNW) * tpw));
+            WDM_LAUNCH((k_cols<T, PP, F, MODE, KF, SNW>), dim3(blocks), dim3(SNW * 64), s, d, ncols, nother, sl, so,
+                       tw, kt, tpw);
+            return true;
+        }
+    }
     template <int MODE, int KF>
     static void col_t(cx<T>* d, int ncols, int nother, unsigned sl, unsigned so, const TwE* tw, KTabs kt,
                       hipStream_t s) {
-        WDM_LAUNCH((k_col<T, PsiP, F, MODE, KF, NW>), dim3(col_blocks<NW, F>(ncols, nother)), dim3(NW * 64), s, d,
-                   ncols, nother, sl, so, tw, kt);
+        if (col_staged<PsiP, MODE, KF>(d, ncols, nother, sl, so, tw, kt, s)) return;
+        if constexpr (WDM_COL_TPW > 1 && MODE != COL_FMIS) {
+            WDM_LAUNCH((k_colp<T, PsiP, F, MODE, KF, NW, WDM_COL_TPW>),
+                       dim3(col_blocks<NW * WDM_COL_TPW, F>(ncols, nother)), dim3(NW * 64), s, d, ncols, nother, sl,
+                       so, tw, kt);
+        } else {
+            constexpr int W = (MODE == COL_FMI && col_min_waves<T, MODE>() > 1) ? WDM_NW_FMI : NW;
+            WDM_LAUNCH((k_col<T, PsiP, F, MODE, KF, W>), dim3(col_blocks<W, F>(ncols, nother)), dim3(W * 64), s,
+                       d, ncols, nother, sl, so, tw, kt);
+        }
     }
     typedef PoissonPolicy PoisP;
     template <int MODE, int KF>
     static void colp_t(cx<T>* d, int ncols, int nother, unsigned sl, unsigned so, const TwE* tw, KTabs kt,
                        hipStream_t s) {
-        WDM_LAUNCH((k_col<T, PoisP, F, MODE, KF, NW>), dim3(col_blocks<NW, F>(ncols, nother)), dim3(NW * 64), s, d,
-                   ncols, nother, sl, so, tw, kt);
+        if (col_staged<PoisP, MODE, KF>(d, ncols, nother, sl, so, tw, kt, s)) return;
+        if constexpr (WDM_COL_TPW > 1) {
+            WDM_LAUNCH((k_colp<T, PoisP, F, MODE, KF, NW, WDM_COL_TPW>),
+                       dim3(col_blocks<NW * WDM_COL_TPW, F>(ncols, nother)), dim3(NW * 64), s, d, ncols, nother, sl,
+                       so, tw, kt);
+        } else {
+            constexpr int W = (MODE == COL_FMI && col_min_waves<T, MODE>() > 1) ? WDM_NW_FMI : NW;
+            WDM_LAUNCH((k_col<T, PoisP, F, MODE, KF, W>), dim3(col_blocks<W, F>(ncols, nother)), dim3(W * 64), s,
+                       d, ncols, nother, sl, so, tw, kt);
+        }
     }
     // mode/kf combinations used by the plan; KF_POISSON passes run in Plain.
     static void col(int mode, int kf, cx<T>* d, int ncols, int nother, unsigned sl, unsigned so, const TwE* tw,
