@@ -601,7 +601,17 @@ struct KickArgs {
 // Kick (+ optional forward x-FFT).  Reads psi rows in real space and Phi_hat
 // (half spectrum after the y/z Poisson passes); writes psi (x-transformed when
 // FWD) in place.
-template <class T, class P, class F, bool FWD, int NW>
+// Potential kinds the kick is compiled for: straight-line code for the
+// common single-model cases, runtime flags for any combination.
+enum VKind { VK_NONE = 0, VK_QUAD = 1, VK_HERN = 2, VK_ANY = 3 };
+inline int vkind_of(int flags) {
+    if (flags == 0) return VK_NONE;
+    if (flags == V_QUADRUPOLE) return VK_QUAD;
+    if (flags == V_HERNQUIST) return VK_HERN;
+    return VK_ANY;
+}
+
+template <class T, class P, class F, bool FWD, int NW, int VK>
 WDM_KERNEL(NW * 64)
 void k_x_kick(cx<T>* __restrict__ psi, const cx<T>* __restrict__ phihat, int nlines, int hpitch,
               const typename TwEntry<T>::type* __restrict__ tw, KickArgs ka) {
@@ -654,9 +664,18 @@ void k_x_kick(cx<T>* __restrict__ psi, const cx<T>* __restrict__ phihat, int nli
         if ((n1 & 3) == 0) WDM_SCHED_FENCE();
         const int x = N2 * n1 + m.j;
         double V = 0.0;
-        if (ka.vm.flags) {
+        if constexpr (VK != VK_NONE) {
             const double tx = double(x) * ka.xinv;
-            V = eval_V_line<T>(ka.vm, lv, (1.0 - tx) * ka.x0 + tx * ka.x1, unsigned(lc) * N + x, v[n1]);
+            const double xc = (1.0 - tx) * ka.x0 + tx * ka.x1;
+            if constexpr (VK == VK_QUAD) {
+                const double dx = xc - ka.vm.q_c[0];
+                V = fmaT(fmaT(lv.qa, dx, lv.qb), dx, lv.qc);
+            } else if constexpr (VK == VK_HERN) {
+                const double dx = xc - ka.vm.hq_c[0];
+                V = -ka.vm.hq_GM / (__builtin_sqrt(fmaT(dx, dx, lv.hr2)) + ka.vm.hq_a);
+            } else {
+                if (ka.vm.flags) V = eval_V_line<T>(ka.vm, lv, xc, unsigned(lc) * N + x, v[n1]);
+            }
         }
         T ph = phi[n1];
         WDM_OPAQUE(ph);

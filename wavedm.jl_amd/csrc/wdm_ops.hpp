@@ -78,7 +78,7 @@ struct AxisOps {
     void (*col)(int mode, int kf, cx<T>* data, int ncols, int nother, unsigned sline, unsigned sother,
                 const TwE* tw, KTabs kt, hipStream_t s);
     void (*x_kick)(bool fwd, cx<T>* psi, const cx<T>* phihat, int nlines, int hpitch, const TwE* tw,
-                   const KickArgs& ka, hipStream_t s);
+                   const KickArgs& ka, hipStream_t s);  // kernel picked by the potential kind of ka.vm
     void (*x_inv_dens)(cx<T>* psi, cx<T>* rhohat, int nlines, int hpitch, const TwE* tw, const double* border,
                        double* partials, hipStream_t s);
     void (*x_dens)(const cx<T>* psi, cx<T>* rhohat, int nlines, int hpitch, const TwE* tw, double* partials,
@@ -178,14 +178,24 @@ struct Launch {
             case COL_MI: col_t<COL_MI, KF_DRIFT>(d, ncols, nother, sl, so, tw, kt, s); break;
         }
     }
-    static void x_kick(bool fwd, cx<T>* psi, const cx<T>* ph, int nlines, int hp, const TwE* tw, const KickArgs& ka,
-                       hipStream_t s) {
+    template <int VK>
+    static void x_kick_v(bool fwd, cx<T>* psi, const cx<T>* ph, int nlines, int hp, const TwE* tw, const KickArgs& ka,
+                         hipStream_t s) {
         if (fwd)
-            WDM_LAUNCH((k_x_kick<T, PsiP, F, true, NW>), dim3(x_blocks<NW, F>(nlines)), dim3(NW * 64), s, psi, ph,
+            WDM_LAUNCH((k_x_kick<T, PsiP, F, true, NW, VK>), dim3(x_blocks<NW, F>(nlines)), dim3(NW * 64), s, psi, ph,
                        nlines, hp, tw, ka);
         else
-            WDM_LAUNCH((k_x_kick<T, PsiP, F, false, NW>), dim3(x_blocks<NW, F>(nlines)), dim3(NW * 64), s, psi, ph,
-                       nlines, hp, tw, ka);
+            WDM_LAUNCH((k_x_kick<T, PsiP, F, false, NW, VK>), dim3(x_blocks<NW, F>(nlines)), dim3(NW * 64), s, psi,
+                       ph, nlines, hp, tw, ka);
+    }
+    static void x_kick(bool fwd, cx<T>* psi, const cx<T>* ph, int nlines, int hp, const TwE* tw, const KickArgs& ka,
+                       hipStream_t s) {
+        switch (vkind_of(ka.vm.flags)) {
+            case VK_NONE: x_kick_v<VK_NONE>(fwd, psi, ph, nlines, hp, tw, ka, s); break;
+            case VK_QUAD: x_kick_v<VK_QUAD>(fwd, psi, ph, nlines, hp, tw, ka, s); break;
+            case VK_HERN: x_kick_v<VK_HERN>(fwd, psi, ph, nlines, hp, tw, ka, s); break;
+            default: x_kick_v<VK_ANY>(fwd, psi, ph, nlines, hp, tw, ka, s); break;
+        }
     }
     static void x_inv_dens(cx<T>* psi, cx<T>* rh, int nlines, int hp, const TwE* tw, const double* border,
                            double* partials, hipStream_t s) {
