@@ -29,11 +29,12 @@ inline LaunchEvents& launch_events() {
 namespace wdm {
 
 // Waves per workgroup of the pass kernels, per precision.  Measured at 384^3
-// on MI355X: FP32 (<= 256 registers per lane) gains from 8-wave groups, whose
-// 64 consecutive columns make 512-byte row segments in the strided passes;
-// FP64 kernels (256 VGPRs + AGPRs, one wave per SIMD) do best with 2.
+// on MI355X (autotuner): FP32 kernels (130-200 registers, two or three waves
+// per SIMD) do best with 4-wave groups, which leave no wave slots unused at
+// three waves per SIMD; FP64 kernels (256 VGPRs + AGPRs, one wave per SIMD)
+// with 2.
 #ifndef WDM_NW_F32
-#define WDM_NW_F32 8
+#define WDM_NW_F32 4
 #endif
 #ifndef WDM_NW_F64
 #define WDM_NW_F64 2
@@ -260,7 +261,9 @@ struct Launch {
     X(192, 48, 4)           \
     X(256, 64, 4)
 
-constexpr int kNWAlt = 4;
+// the other workgroup size tried by the autotuner
+template <class T>
+constexpr int kNWAlt = sizeof(T) == 4 ? 8 : 4;
 constexpr int kMaxVariants = 3;
 
 #define WDM_DECLARE_SIZE(N, A, B)                        \
@@ -282,10 +285,7 @@ WDM_FOR_EACH_ALT(WDM_ALT_SPEC)
 template <class T, class F, int N>
 const AxisOps<T>* axis_variant(int variant) {
     if (variant == 0) return Launch<T, F>::table();
-    if (variant == 1) {
-        if constexpr (kNW<T> == kNWAlt) return nullptr;
-        else return Launch<T, F, kNWAlt>::table();
-    }
+    if (variant == 1) return Launch<T, F, kNWAlt<T>>::table();
     if (variant == 2) {
         typedef typename AltFac<N>::type A;
         if constexpr (std::is_same<A, void>::value) return nullptr;
