@@ -18,7 +18,9 @@
 # is what the tests run.
 module WaveDMHIP
 
-export HIPPlan, set_psi!, get_psi, set_border!, set_static_potential!, set_hernquist!,
+using ChainRulesCore
+
+export HIPPlan, ssfm, set_psi!, get_psi, set_border!, set_static_potential!, set_hernquist!,
        set_quadrupole!, set_kappa!, parallel_poisson, compute_gravitational_potential,
        apply_kick_step!, apply_drift_step!, step!, reduce_diagnostics, cfl_limits
 
@@ -35,6 +37,8 @@ const WDM_V_HERNQUIST = Cint(2)
 const WDM_V_QUADRUPOLE = Cint(4)
 const WDM_V_KAPPA = Cint(8)
 const WDM_RED_COUNT = 6
+const WDM_ADJ_DT_SCALE = Cuint(1)
+const WDM_ADJ_NPARAM = 6
 
 struct WaveDMHIPError <: Exception
     code::Int
@@ -200,6 +204,44 @@ function cfl_limits(p::HIPPlan; κ::Real=0.0)
     out = zeros(Float64, 3)
     check(ccall((:wdm_cfl, libwdm), Cint, (Ptr{Cvoid}, Cdouble, Ptr{Cdouble}), p.h, κ, out), p.h)
     return (out[1], out[2], out[3])
+end
+
+# --------------------------------------------------------------------------
+# Differentiable forward run (the gradient form): the reference imports Zygote
+# (src/WaveDM.jl:42) but defines no rule for the step; this rrule runs the
+# device adjoint (wdm_adjoint_forward / wdm_adjoint_backward).
+
+"""
+    ssfm(p::HIPPlan, ψ0, θ, nsteps, dt; dt_kick=dt/2, ckpt_every=20) -> ψN
+
+`nsteps` reference-split steps from `ψ0` under the Hernquist host
+θ = (M, a, cx, cy, cz) plus the plan's self-gravity, with device checkpoints
+every `ckpt_every` steps.  Differentiable in `ψ0` and `θ` through the rrule
+below (Zygote's complex convention: the cotangent of ψ is ∂L/∂Re ψ + i ∂L/∂Im ψ).
+"""
+function ssfm(p::HIPPlan, ψ0::Array{ComplexF64,3}, θ::NTuple{5,Float64}, nsteps::Integer, dt::Real;
+              dt_kick::Real=dt / 2, ckpt_every::Integer=20)
+    check(ccall((:wdm_clear_potential_models, libwdm), Cint, (Ptr{Cvoid},), p.h), p.h)
+    set_hernquist!(p; M=θ[1], a=θ[2], center=θ[3:5])
+    set_psi!(p, ψ0)
+    check(ccall((:wdm_adjoint_forward, libwdm), Cint, (Ptr{Cvoid}, Cint, Cdouble, Cdouble, Cdouble, Cint),
+                p.h, nsteps, dt, dt_kick, 0.0, ckpt_every), p.h)
+    return get_psi(p)
+end
+
+function ChainRulesCore.rrule(::typeof(ssfm), p::HIPPlan, ψ0::Array{ComplexF64,3}, θ::NTuple{5,Float64},
+                              nsteps::Integer, dt::Real; dt_kick::Real=dt / 2, ckpt_every::Integer=20)
+    ψN = ssfm(p, ψ0, θ, nsteps, dt; dt_kick, ckpt_every)
+    function ssfm_pullback(ΔψN)
+        g = Array{ComplexF64,3}(unthunk(ΔψN))
+        g0 = similar(g)
+        gp = zeros(Float64, WDM_ADJ_NPARAM)
+        check(ccall((:wdm_adjoint_backward, libwdm), Cint,
+                    (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{Cvoid}, Ptr{Cdouble}, Cint, Cuint),
+                    p.h, g, WDM_C128, g0, gp, WDM_ADJ_NPARAM, Cuint(0)), p.h)
+        return NoTangent(), NoTangent(), g0, Tangent{typeof(θ)}(gp[1:5]...), NoTangent(), NoTangent()
+    end
+    return ψN, ssfm_pullback
 end
 
 end # module
