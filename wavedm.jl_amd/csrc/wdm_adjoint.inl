@@ -250,8 +250,8 @@ void adj_conj_drift(wdm_plan* p, cx<T>* g, const cx<T>* Psi, double* part, doubl
                 none, p->stream);
     if (!Psi) {
         ensure_drift(p, A.dt);
-        KTabs kt = {p->ax[0].drift_conj.as<double>(), p->ax[1].drift_conj.as<double>(),
-                    p->ax[2].drift_conj.as<double>(), 1.0 / double(p->ncell), nullptr};
+        KTabs kt = {p->ax[0].drift_conj.p, p->ax[1].drift_conj.p, p->ax[2].drift_conj.p, 1.0 / double(p->ncell),
+                    nullptr};
         oz->col(COL_FMI, KF_NONE, g, p->nx, p->ny, unsigned(p->nx * p->ny), unsigned(p->nx), p->ax[2].tw.as<TwE>(), kt,
                 p->stream);
     } else {

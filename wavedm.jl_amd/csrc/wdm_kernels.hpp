@@ -40,34 +40,86 @@ enum ColMode { COL_FWD = 0, COL_INV = 1, COL_FMI = 2, COL_MI = 3, COL_FMIS = 4 }
 enum KFactor { KF_NONE = 0, KF_DRIFT = 1, KF_POISSON = 2, KF_PNONE = 3 };
 
 // k-space factor tables for the line, column and "other" axes of a pass.
-//   KF_DRIFT:   complex FP64 exp(-i (hbar/m) k_a^2 dt / 2) per axis (interleaved),
-//               multiplied together and by `scale` (= 1/N).
-//   KF_POISSON: real FP64 k_a^2 per axis; factor = scale / (kc^2+ko^2+kl^2),
-//               0 at k = 0 (scale = -4 pi G / N).
+//   KF_DRIFT:   complex FP64 exp(-i (hbar/m) k_a^2 dt / 2) per axis,
+//               multiplied together and by `scale` (= 1/N).  The column x other
+//               product is formed once per lane; each element's full factor is
+//               formed in FP64 and, in FP32 plans, split into hi/lo for one
+//               compensated multiply.  (Two compensated FP32 multiplies, by the
+//               line factor and then by the lane factor, bias |psi|^2 by
+//               ~1e-10 per step where cos(k^2 dt) rounds to 1.0f.)
+//   KF_POISSON: k_a^2 per axis (FP64, or FP32 in FP32 plans); factor =
+//               scale / (kc^2+ko^2+kl^2), 0 at k = 0 (scale = -4 pi G / N).
 struct KTabs {
-    const double* col;
-    const double* oth;
-    const double* line;
+    const void* col;
+    const void* oth;
+    const void* line;
     double scale;
     void* spec_out;  // COL_FMIS: the forward spectrum (before the factor) is
                      // also stored here, same layout as data
 };
 
-template <int KF, class P, class T>
-WDM_FI cx<T> apply_kf(cx<T> v, const KTabs& kt, int c, int o, int k) {
+// line-axis table entry of a factor kind
+template <int KF, class T>
+struct LineEntry {
+    typedef char type;  // no table
+};
+template <class T>
+struct LineEntry<KF_DRIFT, T> {
+    typedef cx<double> type;
+};
+template <class T>
+struct LineEntry<KF_POISSON, T> {
+    typedef T type;
+};
+
+// the per-lane part of a factor (column and other axes)
+template <int KF, class T>
+struct KfLane {
+    int unused;
+};
+template <class T>
+struct KfLane<KF_DRIFT, T> {
+    double ar, ai;  // col * oth * scale
+};
+template <class T>
+struct KfLane<KF_POISSON, T> {
+    T kco;  // kc^2 + ko^2
+    T scale;
+};
+
+template <int KF, class T>
+WDM_FI KfLane<KF, T> kf_lane(const KTabs& kt, int c, int o) {
+    KfLane<KF, T> l;
     if constexpr (KF == KF_DRIFT) {
-        const double ar = kt.col[2 * c], ai = kt.col[2 * c + 1];
-        const double br = kt.oth[2 * o], bi = kt.oth[2 * o + 1];
-        const double cr = kt.line[2 * k], ci = kt.line[2 * k + 1];
-        const double abr = ar * br - ai * bi, abi = ar * bi + ai * br;
-        const double re = (abr * cr - abi * ci) * kt.scale;
-        const double im = (abr * ci + abi * cr) * kt.scale;
-        return mul_dyn<P>(v, re, im);
+        const cx<double> a = static_cast<const cx<double>*>(kt.col)[c];
+        const cx<double> b = static_cast<const cx<double>*>(kt.oth)[o];
+        l.ar = (a.x * b.x - a.y * b.y) * kt.scale;
+        l.ai = (a.x * b.y + a.y * b.x) * kt.scale;
     } else if constexpr (KF == KF_POISSON) {
-        const double s = kt.col[c] + kt.oth[o] + kt.line[k];
-        const double g = s > 0.0 ? kt.scale / s : 0.0;
-        return v * T(g);
+        l.kco = static_cast<const T*>(kt.col)[c] + static_cast<const T*>(kt.oth)[o];
+        l.scale = T(kt.scale);
     } else {
+        (void)kt; (void)c; (void)o;
+    }
+    return l;
+}
+
+template <int KF, class P, class T>
+WDM_FI cx<T> apply_kf(cx<T> v, const KfLane<KF, T>& l, const KTabs& kt, int k) {
+    if constexpr (KF == KF_DRIFT) {
+        const cx<double> c = static_cast<const cx<double>*>(kt.line)[k];
+        return mul_dyn<P>(v, l.ar * c.x - l.ai * c.y, l.ar * c.y + l.ai * c.x);
+    } else if constexpr (KF == KF_POISSON) {
+        const T s = l.kco + static_cast<const T*>(kt.line)[k];
+        if constexpr (std::is_same<T, double>::value) {
+            const double g = s > 0.0 ? l.scale / s : 0.0;
+            return v * g;
+        } else {
+            const float g = s > 0.0f ? l.scale * rcpf(s) : 0.0f;
+            return v * g;
+        }
+    } else {
+        (void)l; (void)kt; (void)k;
         return v;
     }
 }
@@ -364,7 +416,7 @@ void k_cols(cx<T>* __restrict__ data, int ncols, int nother, unsigned sline, uns
         if (more) fetch(tn);
         if constexpr (MODE == COL_MI) {
             WDM_UNROLL
-            for (int n1 = 0; n1 < N1; ++n1) v[n1] = apply_kf<KF, P, T>(v[n1], kt, cc, o, N2 * n1 + m.j);
+            for (int n1 = 0; n1 < N1; ++n1) v[n1] = apply_kf<KF, P, T>(v[n1], kf_lane<KF, T>(kt, cc, o), kt, N2 * n1 + m.j);
         }
         cx<T> X[M][N2];
         fwd2<F, DIR, P>(v, X, m.j, xb, tw);
@@ -374,7 +426,7 @@ void k_cols(cx<T>* __restrict__ data, int ncols, int nother, unsigned sline, uns
                 WDM_SCHED_FENCE();
                 WDM_UNROLL
                 for (int k2 = 0; k2 < N2; ++k2)
-                    X[i][k2] = apply_kf<KF, P, T>(X[i][k2], kt, cc, o, (i * N2 + m.j) + N1 * k2);
+                    X[i][k2] = apply_kf<KF, P, T>(X[i][k2], kf_lane<KF, T>(kt, cc, o), kt, (i * N2 + m.j) + N1 * k2);
             }
             inv2<F, +1, P>(X, v, m.j, xb, tw);
             if (ok) {
@@ -452,7 +504,7 @@ void k_colp(cx<T>* __restrict__ data, int ncols, int nother, unsigned sline, uns
         WDM_SCHED_FENCE();
         if constexpr (MODE == COL_MI) {
             WDM_UNROLL
-            for (int n1 = 0; n1 < N1; ++n1) v[n1] = apply_kf<KF, P, T>(v[n1], kt, cc, oc, N2 * n1 + m.j);
+            for (int n1 = 0; n1 < N1; ++n1) v[n1] = apply_kf<KF, P, T>(v[n1], kf_lane<KF, T>(kt, cc, oc), kt, N2 * n1 + m.j);
         }
         cx<T> X[M][N2];
         fwd2<F, DIR, P>(v, X, m.j, xb, tw);
@@ -462,7 +514,7 @@ void k_colp(cx<T>* __restrict__ data, int ncols, int nother, unsigned sline, uns
                 WDM_SCHED_FENCE();
                 WDM_UNROLL
                 for (int k2 = 0; k2 < N2; ++k2)
-                    X[i][k2] = apply_kf<KF, P, T>(X[i][k2], kt, cc, oc, (i * N2 + m.j) + N1 * k2);
+                    X[i][k2] = apply_kf<KF, P, T>(X[i][k2], kf_lane<KF, T>(kt, cc, oc), kt, (i * N2 + m.j) + N1 * k2);
             }
             inv2<F, +1, P>(X, v, m.j, xb, tw);
             if (ok) {
@@ -503,9 +555,15 @@ void k_colp(cx<T>* __restrict__ data, int ncols, int nother, unsigned sline, uns
 #ifndef WDM_MINW_XINV_F32
 #define WDM_MINW_XINV_F32 1
 #endif
+#ifndef WDM_MINW_COL_F32
+#define WDM_MINW_COL_F32 1
+#endif
 template <class T, int MODE>
 constexpr int col_min_waves() {
-    return (std::is_same<T, float>::value && (MODE == COL_FMI)) ? WDM_MINW_FMI_F32 : 1;
+    if (!std::is_same<T, float>::value) return 1;
+    if (MODE == COL_FMI) return WDM_MINW_FMI_F32;
+    if (MODE == COL_FWD || MODE == COL_INV) return WDM_MINW_COL_F32;
+    return 1;
 }
 
 template <class T, class P, class F, int MODE, int KF, int NW>
@@ -515,11 +573,13 @@ void k_col(cx<T>* __restrict__ data, int ncols, int nother, unsigned sline, unsi
     constexpr int N1 = F::N1, N2 = F::N2, M = F::M;
     WDM_SHARED cx<T> xbuf[NW][F::XBUF];
     WDM_SHARED typename TwEntry<T>::type tw_s[F::N];
-    // k-space factor of the line axis (complex drift factors or real k^2)
-    constexpr int NL = KF == KF_DRIFT ? 2 * F::N : (KF == KF_POISSON ? F::N : 1);
-    WDM_SHARED double line_s[NL];
+    // k-space factor table of the line axis, staged with the twiddles
+    typedef typename LineEntry<KF, T>::type LE;
+    constexpr int NL = (KF == KF_DRIFT || KF == KF_POISSON) ? F::N : 1;
+    WDM_SHARED LE line_s[NL];
     if constexpr (KF == KF_DRIFT || KF == KF_POISSON) {
-        for (int i = int(threadIdx.x); i < NL; i += int(blockDim.x)) line_s[i] = kt.line[i];
+        const LE* src = static_cast<const LE*>(kt.line);
+        for (int i = int(threadIdx.x); i < NL; i += int(blockDim.x)) line_s[i] = src[i];
         kt.line = line_s;
     }
     tw = stage_table<F::N>(tw_s, tw);
@@ -529,6 +589,7 @@ void k_col(cx<T>* __restrict__ data, int ncols, int nother, unsigned sline, unsi
     const int c = tile * F::LPW + m.lw;
     const bool ok = (o < nother) && (c < ncols);
     const int oc = ok ? o : 0, cc = ok ? c : 0;
+    const KfLane<KF, T> kl = kf_lane<KF, T>(kt, cc, oc);
     // 32-bit element offsets from the uniform base pointer: one VGPR per
     // address (scalar-base addressing) and cheap to recompute for the stores
     const unsigned base = unsigned(oc) * sother + unsigned(cc) + unsigned(m.j) * sline;
@@ -541,7 +602,7 @@ void k_col(cx<T>* __restrict__ data, int ncols, int nother, unsigned sline, unsi
         WDM_UNROLL
         for (int n1 = 0; n1 < N1; ++n1) {
             const int k = N2 * n1 + m.j;
-            v[n1] = apply_kf<KF, P, T>(WDM_LDG(ok, data, base + unsigned(n1) * step), kt, cc, oc, k);
+            v[n1] = apply_kf<KF, P, T>(WDM_LDG(ok, data, base + unsigned(n1) * step), kl, kt, k);
         }
     } else {
         WDM_UNROLL
@@ -564,7 +625,7 @@ void k_col(cx<T>* __restrict__ data, int ncols, int nother, unsigned sline, unsi
             WDM_SCHED_FENCE();
             WDM_UNROLL
             for (int k2 = 0; k2 < N2; ++k2)
-                X[i][k2] = apply_kf<KF, P, T>(X[i][k2], kt, cc, oc, (i * N2 + m.j) + N1 * k2);
+                X[i][k2] = apply_kf<KF, P, T>(X[i][k2], kl, kt, (i * N2 + m.j) + N1 * k2);
         }
         inv2<F, +1, P>(X, v, m.j, xb, tw);
         if (ok) {

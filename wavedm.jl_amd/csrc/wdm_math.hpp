@@ -133,6 +133,17 @@ struct TwTab {
 template <int R>
 inline constexpr TwTab<R> kTw{};
 
+// ---------------------------------------------------------------- reciprocal
+// FP32 reciprocal to about 1 ulp (v_rcp_f32); used where a relative error of
+// a few 1e-7 is below the arithmetic's own (Poisson factor of FP32 plans).
+WDM_FI float rcpf(float x) {
+#ifdef WDM_GPU
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+
 // ---------------------------------------------------------------- hi/lo splits
 // v = hi + lo, hi = float(v), lo = float(v - hi).
 struct HL {

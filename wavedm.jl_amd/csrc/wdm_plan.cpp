@@ -87,7 +87,7 @@ struct AxisState {
     DBuf coord;                 // FP64 coordinates
     DBuf drift;                 // complex FP64 drift factors for dt_cached
     DBuf drift_conj;            // their complex conjugates (adjoint)
-    DBuf k2;                    // FP64 k^2 (FFT order; x axis: half spectrum)
+    DBuf k2;                    // k^2 (FFT order; x axis: half spectrum), FP64 or FP32
 };
 
 // Start/stop HIP events attached to each pass launch (wdm_step_timed).
@@ -236,7 +236,13 @@ void build_k2(wdm_plan* p) {
             const double k = 2.0 * M_PI * (a == 0 ? m : fft_int(n, m)) / L;
             h[m] = k * k;
         }
-        upload(p->ax[a].k2, h);
+        if (p->precision == WDM_F32) {
+            std::vector<float> f(h.begin(), h.end());
+            float* d = p->ax[a].k2.get<float>(f.size());
+            hip_check(hipMemcpy(d, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice), "k2 upload");
+        } else {
+            upload(p->ax[a].k2, h);
+        }
     }
 }
 
@@ -263,7 +269,7 @@ void poisson_chain(wdm_plan* p, double scale) {
         mark(p, WDM_PASS_PYFWD);
     }
     // Phi_k = -scale * rho_k / k^2, with the 1/N of the unnormalised inverse folded in
-    KTabs kt = {p->ax[0].k2.as<double>(), p->ax[1].k2.as<double>(), p->ax[2].k2.as<double>(),
+    KTabs kt = {p->ax[0].k2.p, p->ax[1].k2.p, p->ax[2].k2.p,
                 -scale / double(p->ncell), nullptr};
     arm(p);
     oz->col(COL_FMI, KF_POISSON, rh, nh, p->ny, unsigned(hp * p->ny), unsigned(hp), p->ax[2].tw.as<TwE>(), kt,
@@ -416,8 +422,8 @@ void z_pass(wdm_plan* p, int mode, double dt) {
     KTabs kt = {nullptr, nullptr, nullptr, 0.0, nullptr};
     if (mode == COL_FMI || mode == COL_MI) {
         ensure_drift(p, dt);
-        kt = {p->ax[0].drift.as<double>(), p->ax[1].drift.as<double>(), p->ax[2].drift.as<double>(),
-              1.0 / double(p->ncell), mode == COL_FMI ? p->zspec_out : nullptr};
+        kt = {p->ax[0].drift.p, p->ax[1].drift.p, p->ax[2].drift.p, 1.0 / double(p->ncell),
+              mode == COL_FMI ? p->zspec_out : nullptr};
         if (mode == COL_FMI && p->zspec_out) mode = COL_FMIS;
     } else if (p->nz == 1) {
         return;
