@@ -22,12 +22,19 @@
 namespace {
 
 // ------------------------------------------------------------------ kernels
-WDM_FI void cell_xyz(size_t i, int nx, int ny, int& x, int& y, int& z) {
-    x = int(i % size_t(nx));
-    const size_t r = i / size_t(nx);
-    y = int(r % size_t(ny));
-    z = int(r / size_t(ny));
-}
+// Elementwise kernels: workgroup b takes the cells of lines [b*kLPB, (b+1)*kLPB)
+// (line = y + ny*z), its threads stride over them; 32-bit index arithmetic
+// only (plans are below 2^31 cells).
+constexpr int kLPB = 8;
+inline unsigned adj_line_blocks(int ny, int nz) { return unsigned((ny * nz + kLPB - 1) / kLPB); }
+#define WDM_FOR_CELLS(nx, ny, nz)                                                                     \
+    for (unsigned j_ = threadIdx.x, lines_ = unsigned((ny) * (nz)), l0_ = blockIdx.x * unsigned(kLPB); \
+         j_ < unsigned(kLPB) * unsigned(nx); j_ += blockDim.x)                                         \
+        if (const unsigned line_ = l0_ + j_ / unsigned(nx); line_ < lines_)                           \
+            if (const int x = int(j_ % unsigned(nx)), y = int(line_ % unsigned(ny)),                  \
+                z = int(line_ / unsigned(ny));                                                        \
+                true)                                                                                  \
+                if (const unsigned i = line_ * unsigned(nx) + unsigned(x); true)
 
 WDM_FI cx<double> expmi(double theta) {
     int q;
@@ -63,12 +70,9 @@ void k_adj_s(const cx<T>* __restrict__ g, const cx<T>* __restrict__ psi, const d
              double* __restrict__ partials) {
     WDM_SHARED double sh[kAdjNV][256];
     double acc[kAdjNV] = {0, 0, 0, 0, 0, 0};
-    const size_t n = size_t(nx) * ny * nz;
-    for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) {
-        int x, y, z;
-        cell_xyz(i, nx, ny, x, y, z);
+    WDM_FOR_CELLS(nx, ny, nz) {
         const cx<T> pv = psi[i];
-        const double V = vm.flags ? eval_V<T>(vm, xs[x], ys[y], zs[z], unsigned(i), pv) : 0.0;
+        const double V = vm.flags ? eval_V<T>(vm, xs[x], ys[y], zs[z], i, pv) : 0.0;
         const double ph = (phi ? phi[i] : 0.0) + V;
         const cx<double> e = expmi(tau * ph);
         const double pr = double(pv.x), pi = double(pv.y);
@@ -102,12 +106,9 @@ void k_adj_g(cx<T>* __restrict__ g, const cx<T>* __restrict__ psi, const double*
              const double* __restrict__ u, const double* __restrict__ s, int nx, int ny, int nz,
              const double* __restrict__ xs, const double* __restrict__ ys, const double* __restrict__ zs, VModel vm,
              double tau, const double* __restrict__ border) {
-    const size_t n = size_t(nx) * ny * nz;
-    for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) {
-        int x, y, z;
-        cell_xyz(i, nx, ny, x, y, z);
+    WDM_FOR_CELLS(nx, ny, nz) {
         const cx<T> pv = psi[i];
-        const double V = vm.flags ? eval_V<T>(vm, xs[x], ys[y], zs[z], unsigned(i), pv) : 0.0;
+        const double V = vm.flags ? eval_V<T>(vm, xs[x], ys[y], zs[z], i, pv) : 0.0;
         const cx<double> e = expmi(tau * ((phi ? phi[i] : 0.0) + V));
         const double gr = double(g[i].x), gi = double(g[i].y);
         double w = u ? u[i] : 0.0;
@@ -131,10 +132,7 @@ void k_adj_spec(cx<T>* __restrict__ G, const cx<T>* __restrict__ Psi, int nx, in
                 double c /* (hbar/m) dt / 2 */, double inv_n, double* __restrict__ partials) {
     WDM_SHARED double sh[1][256];
     double acc[1] = {0.0};
-    const size_t n = size_t(nx) * ny * nz;
-    for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) {
-        int x, y, z;
-        cell_xyz(i, nx, ny, x, y, z);
+    WDM_FOR_CELLS(nx, ny, nz) {
         const double w = c * (kx2[x] + ky2[y] + kz2[z]);
         const cx<double> e = expmi(w);  // P; conj(P) = (e.x, -e.y)
         const double gr = double(G[i].x), gi = double(G[i].y);
@@ -204,7 +202,7 @@ AdjState& adj_of(wdm_plan* p) {
     return *p->adj;
 }
 
-unsigned adj_blocks(size_t n) { return unsigned(std::min<size_t>((n + 255) / 256, 2048)); }
+
 
 template <class T>
 void adj_forward(wdm_plan* p, int nsteps, double dt, double dt_kick, double t0, int K) {
@@ -233,10 +231,10 @@ void adj_forward(wdm_plan* p, int nsteps, double dt, double dt_kick, double t0, 
     A.ready = true;
 }
 
-// g (device) <- U^H g, the conjugate drift; with Psi given, also the dt-scale
-// spectral term into acc[5] (unfused passes)
+// g (device) <- U^H g, the conjugate drift; with Psi given, also the block
+// partials of the dt-scale spectral term into part (unfused passes)
 template <class T>
-void adj_conj_drift(wdm_plan* p, cx<T>* g, const cx<T>* Psi, double* part, double* acc) {
+void adj_conj_drift(wdm_plan* p, cx<T>* g, const cx<T>* Psi, double* part) {
     typedef typename TwEntry<T>::type TwE;
     AdjState& A = adj_of(p);
     const AxisOps<T>* ox = ops_of<T>(p, 0);
@@ -258,11 +256,10 @@ void adj_conj_drift(wdm_plan* p, cx<T>* g, const cx<T>* Psi, double* part, doubl
         if (p->nz > 1)
             oz->col(COL_FWD, KF_NONE, g, p->nx, p->ny, unsigned(p->nx * p->ny), unsigned(p->nx),
                     p->ax[2].tw.as<TwE>(), none, p->stream);
-        const unsigned nb = adj_blocks(p->ncell);
+        const unsigned nb = adj_line_blocks(p->ny, p->nz);
         WDM_LAUNCH((k_adj_spec<T>), dim3(nb), dim3(256), p->stream, g, Psi, p->nx, p->ny, p->nz,
                    A.kfull[0].as<double>(), A.kfull[1].as<double>(), A.kfull[2].as<double>(),
                    0.5 * p->hbar_over_m * A.dt, 1.0 / double(p->ncell), part);
-        WDM_LAUNCH((k_adj_accumulate), dim3(1), dim3(256), p->stream, (const double*)part, int(nb), 1, acc + 5);
         if (p->nz > 1)
             oz->col(COL_INV, KF_NONE, g, p->nx, p->ny, unsigned(p->nx * p->ny), unsigned(p->nx),
                     p->ax[2].tw.as<TwE>(), none, p->stream);
@@ -286,14 +283,18 @@ void adj_backward(wdm_plan* p, const void* grad, int dtype, void* grad0, double*
     const size_t n = p->ncell;
     const AxisOps<T>* ox = ops_of<T>(p, 0);
     const int nlines = p->ny * p->nz;
-    const unsigned nb = adj_blocks(n);
+    const unsigned nb = adj_line_blocks(p->ny, p->nz);
     cx<T>* ck = A.ckpt.as<cx<T>>();
     cx<T>* tpsi = A.tpsi.get<cx<T>>(n * size_t(A.K));
     double* tphi = p->self_gravity ? A.tphi.get<double>(n * size_t(A.K)) : nullptr;
     cx<T>* g = A.g.get<cx<T>>(n);
     double* s = A.s.get<double>(n);
     double* u = p->self_gravity ? A.u.get<double>(n) : nullptr;
-    double* part = A.part.get<double>(size_t(kAdjNV) * nb);
+    // block partials of every step, summed once at the end in a fixed order
+    // (the result does not depend on the checkpoint interval):
+    // [nsteps][kAdjNV][nb] parameter sums, then [nsteps][nb] dt-scale spectral sums
+    double* part = A.part.get<double>(size_t(A.nsteps) * (kAdjNV + 1) * nb);
+    double* part_spec = part + size_t(A.nsteps) * kAdjNV * nb;
     double* acc = A.acc.get<double>(kAdjNV);
     cx<T>* tspec = want_dt ? A.tspec.get<cx<T>>(n * size_t(A.K)) : nullptr;
     if (want_dt) {
@@ -356,11 +357,9 @@ void adj_backward(wdm_plan* p, const void* grad, int dtype, void* grad0, double*
             const cx<T>* pn = tpsi + n * size_t(i);
             const double* fn = tphi ? tphi + n * size_t(i) : nullptr;
             const VModel vm = make_vmodel(p, A.t0 + double(step) * A.dt);
-            adj_conj_drift<T>(p, g, tspec ? tspec + n * size_t(i) : nullptr, part, acc);
+            adj_conj_drift<T>(p, g, tspec ? tspec + n * size_t(i) : nullptr, part_spec + size_t(step) * nb);
             WDM_LAUNCH((k_adj_s<T>), dim3(nb), dim3(256), p->stream, (const cx<T>*)g, pn, fn, s, p->nx, p->ny, p->nz,
-                       xs, ys, zs, vm, tau, part);
-            WDM_LAUNCH((k_adj_accumulate), dim3(kAdjNV), dim3(256), p->stream, (const double*)part, int(nb), kAdjNV,
-                       acc);
+                       xs, ys, zs, vm, tau, part + size_t(step) * kAdjNV * nb);
             if (u) {
                 ox->x_r2c_real(s, p->rhohat.as<cx<T>>(), nlines, p->hpitch, p->ax[0].tw.as<TwE>(), p->stream);
                 poisson_chain<T>(p, p->four_pi_G);
@@ -371,6 +370,11 @@ void adj_backward(wdm_plan* p, const void* grad, int dtype, void* grad0, double*
             hip_check(hipGetLastError(), "adjoint step");
         }
     }
+    WDM_LAUNCH((k_adj_accumulate), dim3(kAdjNV), dim3(256), p->stream, (const double*)part, int(A.nsteps * nb),
+               kAdjNV, acc);
+    if (want_dt)
+        WDM_LAUNCH((k_adj_accumulate), dim3(1), dim3(256), p->stream, (const double*)part_spec, int(A.nsteps * nb), 1,
+                   acc + 5);
     // outputs; restore psi_N
     double h[kAdjNV];
     hip_check(hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, p->stream), "param grads");
