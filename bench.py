@@ -95,8 +95,19 @@ def run_gpu(W, cfg, precision, steps, warmup):
         p.synchronize()
         wall = time.perf_counter() - t0
         red1 = p.reduce()
+        # the reference loop's per-step diagnostics on the final state (device)
+        diag_ms = {}
+        for label, nslots in (("full", W._capi.WDM_DIAG_COUNT), ("without_potential", W._capi.WDM_DIAG_PE_ABS)):
+            ts = []
+            for _ in range(3):
+                t1 = time.perf_counter()
+                diag = p.diagnostics(nslots)
+                ts.append(time.perf_counter() - t1)
+            diag_ms[label] = 1e3 * sorted(ts)[1]
+        diag = p.diagnostics()
         psi = p.get_psi(np.complex128)
-    return dict(dt=dt, cfl=lims, wall=wall, event_ms=ev_ms, passes=passes, red0=red0, red1=red1, psi=psi)
+    return dict(dt=dt, cfl=lims, wall=wall, event_ms=ev_ms, passes=passes, red0=red0, red1=red1, psi=psi,
+                diag=diag, diag_ms=diag_ms)
 
 
 def cpu_baseline(W, cfg, dt, nsteps):
@@ -183,6 +194,7 @@ def adjoint_leg(W, steps=None):
                 four_pi_G=cfg["four_pi_G"]) as p:
         W.configs.install(p, cfg)
         dt, _ = W.configs.autoset_dt(p, nsteps)
+        p.autotune(3, dt)   # pass tables for this grid (kept state); forward and backward share them
         psi0 = p.get_psi(np.complex64)
         hq = cfg["potential"][1]
         loss = W.adjoint.ProfileLoss(cfg["shape"], cfg["xmax"], nbins=40)
@@ -212,10 +224,16 @@ def adjoint_leg(W, steps=None):
         t0 = time.perf_counter()
         g0, gp = p.adjoint_backward(gN.astype(np.complex64), dt_scale=True)
         t_bwd = time.perf_counter() - t0
+        # the same sweep without the dt-scale spectral term (fused conjugate drift)
+        t0 = time.perf_counter()
+        p.adjoint_backward(gN.astype(np.complex64), dt_scale=False)
+        t_bwd_nodt = time.perf_counter() - t0
     return {"workload": f"c4: FP32 128^3, 16 solitons + Hernquist host, {nsteps} steps, checkpoints every 20, "
                         "40-bin radial profile loss, gradients w.r.t. psi0 and (M, a, centre, dt-scale)",
             "forward_ms": 1e3 * t_fwd, "taped_forward_ms": 1e3 * t_tape, "backward_ms": 1e3 * t_bwd,
             "backward_over_forward": t_bwd / t_fwd, "gradient_over_forward": (t_tape + t_bwd) / t_fwd,
+            "backward_ms_without_dt_scale": 1e3 * t_bwd_nodt,
+            "backward_over_forward_without_dt_scale": t_bwd_nodt / t_fwd,
             "loss": L, "grad_params": dict(zip(W.adjoint.PARAM_NAMES, [float(v) for v in gp])),
             "grad_psi0_norm": float(np.linalg.norm(g0))}
 
@@ -326,6 +344,9 @@ def main():
     m0, m1 = r32["red0"]["mass"], r32["red1"]["mass"]
     out["fp32"] = {"mass_drift": abs(m1 - m0) / m0, "energy0": energy(r32["red0"]),
                    "energy1": energy(r32["red1"])}
+    out["diagnostics"] = {"ms_per_call": r32["diag_ms"], "values": r32["diag"],
+                          "note": "wdm_diagnostics on the final FP32 state: mass, peak, R, R1..R9, momenta, "
+                                  "PE_abs/KE/QE/Virial (full includes the Phi_all solve)"}
     if not args.skip_fp64:
         print("[bench] FP64 leg", file=sys.stderr, flush=True)
         r64 = run_gpu(W, cfg, "f64", args.steps, min(args.warmup, 5))

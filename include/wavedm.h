@@ -34,6 +34,9 @@
  *   wdm_cfl              compute_timestep limits           src/physics/poisson.jl:35-50
  *   wdm_reduce           mass / energy diagnostics         src/simulation_main.jl:552-558,
  *                                                          src/visualization/runtime.jl:176-179
+ *   wdm_diagnostics      the loop's per-step diagnostics   src/simulation_main.jl:550-558,641-656,
+ *                        (Prop.csv columns)                src/visualization/runtime.jl:210-238
+ *   wdm_acceleration     grad_central(-dx, Phi_all)        src/simulation_main.jl:652
  */
 #ifndef WAVEDM_H
 #define WAVEDM_H
@@ -193,6 +196,40 @@ int wdm_adjoint_backward(wdm_plan* plan, const void* grad_psi, int dtype, void* 
 #define WDM_MAX_VARIANTS 3
 int wdm_autotune(wdm_plan* plan, int trial_steps, double dt, int* variants_out, double* ms_out);
 int wdm_select_variants(wdm_plan* plan, const int* variants);
+
+/* Per-step diagnostics of the reference loop, evaluated on the device from the
+   plan's current psi (src/simulation_main.jl:550-558,641-656; the virial terms
+   and momenta of src/visualization/runtime.jl:210-238; the Prop.csv columns of
+   src/io/save.jl:41-72).  rho = |psi|^2, dV = dx dy dz, Phi_all = self-gravity
+   Phi of the current psi + the external potential (static, Hernquist,
+   quadrupole at the last kick time; not the kappa term).  Sums marked "int"
+   run over the interior cells 1..n-2 of every axis (0-based).
+   out[0..min(nout, WDM_DIAG_COUNT)); asking for fewer than
+   WDM_DIAG_PE_ABS + 1 slots skips computing Phi. */
+#define WDM_DIAG_MASS 0      /* TotalMass = sum rho dV                                  */
+#define WDM_DIAG_RHOMAX 1    /* max rho                                                 */
+#define WDM_DIAG_PEAK_IX 2   /* 0-based cell of the first maximum (x-fastest order)     */
+#define WDM_DIAG_PEAK_IY 3
+#define WDM_DIAG_PEAK_IZ 4
+#define WDM_DIAG_R 5         /* std(|x| on plane z = nz/2-1, weights rho), uncorrected  */
+#define WDM_DIAG_R1 6        /* R1..R9: rho-weighted quantiles 0.1..0.9 of |x - x_peak| */
+#define WDM_DIAG_MOMX 15     /* sum Im(conj psi d psi/dx) dV (central differences,      */
+#define WDM_DIAG_MOMY 16     /*   periodic neighbours on the faces), likewise y, z      */
+#define WDM_DIAG_MOMZ 17
+#define WDM_DIAG_PE_ABS 18   /* -1/2 sum_int rho Phi_all dV                             */
+#define WDM_DIAG_KE 19       /* 1/2 sum_int |Im(conj psi grad psi)|^2 / rho dV          */
+#define WDM_DIAG_QE 20       /* 1/2 sum_int |grad |psi||^2 dV                           */
+#define WDM_DIAG_VIRIAL 21   /* 2 KE + 2 QE - PE_abs                                    */
+#define WDM_DIAG_COUNT 22
+/* The radii come from a weighted histogram of |x - x_peak| with WDM_DIAG_BINS
+   bins over [0, distance to the farthest corner], linear inside the bin that
+   holds the quantile; the reference sorts every cell (StatsBase quantile). */
+#define WDM_DIAG_BINS 4096
+int wdm_diagnostics(wdm_plan* plan, double* out, int nout);
+/* -grad Phi_all by central differences (grad_central(-dx, -dy, -dz, Phi_all),
+   src/simulation_main.jl:652; periodic neighbours on the faces) into three
+   host arrays of nx*ny*nz doubles, x-fastest. */
+int wdm_acceleration(wdm_plan* plan, double* host_ax, double* host_ay, double* host_az);
 
 /* Batched ensemble of independent plans (uneven grids; no counterpart in the
    reference, which steps one box at a time).  The plans stay owned by the

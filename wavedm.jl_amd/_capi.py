@@ -43,6 +43,18 @@ WDM_RED_PHIMAX = 5
 WDM_RED_COUNT = 6
 WDM_ADJ_DT_SCALE = 1
 WDM_ADJ_NPARAM = 6
+WDM_DIAG_MASS = 0
+WDM_DIAG_RHOMAX = 1
+WDM_DIAG_PEAK_IX = 2
+WDM_DIAG_R = 5
+WDM_DIAG_R1 = 6
+WDM_DIAG_MOMX = 15
+WDM_DIAG_PE_ABS = 18
+WDM_DIAG_KE = 19
+WDM_DIAG_QE = 20
+WDM_DIAG_VIRIAL = 21
+WDM_DIAG_COUNT = 22
+WDM_DIAG_BINS = 4096
 WDM_MAX_VARIANTS = 3
 
 # every symbol include/wavedm.h declares: name -> (restype, argtypes)
@@ -71,6 +83,8 @@ SIGNATURES = {
     "wdm_drift": (_I, [_P, _D]),
     "wdm_step": (_I, [_P, _I, _D, _D, _D, _I]),
     "wdm_reduce": (_I, [_P, _DP, _I]),
+    "wdm_diagnostics": (_I, [_P, _DP, _I]),
+    "wdm_acceleration": (_I, [_P, _DP, _DP, _DP]),
     "wdm_cfl": (_I, [_P, _D, _DP]),
     "wdm_synchronize": (_I, [_P]),
     "wdm_step_timed": (_I, [_P, _I, _D, _D, _D, _DP, _I, _DP]),

@@ -36,6 +36,13 @@ void launch(dim3 grid, dim3 block, const std::function<void()>& body);
 #define gridDim (emu::g_ctx.gdim)
 #define __syncthreads() emu::barrier()
 
+// Fibers switch only at barriers, so a read-modify-write is atomic here.
+inline unsigned long long atomicAdd(unsigned long long* a, unsigned long long v) {
+    const unsigned long long o = *a;
+    *a = o + v;
+    return o;
+}
+
 // ---- fake runtime API (host memory stands in for device memory) ----------
 typedef int hipError_t;
 typedef void* hipStream_t;

@@ -127,6 +127,7 @@ struct wdm_plan {
     hipStream_t stream = nullptr;
     AxisState ax[3];
     DBuf psi, rhohat, partials, redout, border, vstatic, scratch, scratch_real;
+    DBuf diag;  // diagnostics partials and histograms
     size_t npartials = 0;
     bool border_on = false;
     // state
@@ -1015,7 +1016,7 @@ int wdm_plan_destroy(wdm_plan* p) {
         a.k2.reset();
     }
     for (DBuf* b : {&p->psi, &p->rhohat, &p->partials, &p->redout, &p->border, &p->vstatic, &p->scratch,
-                    &p->scratch_real})
+                    &p->scratch_real, &p->diag})
         b->reset();
     if (p->stream) hipStreamDestroy(p->stream);
     delete p;
@@ -1225,3 +1226,4 @@ int wdm_synchronize(wdm_plan* p) {
 
 #include "wdm_adjoint.inl"
 #include "wdm_ensemble.inl"
+#include "wdm_diag.inl"

@@ -22,7 +22,8 @@ using ChainRulesCore
 
 export HIPPlan, ssfm, set_psi!, get_psi, set_border!, set_static_potential!, set_hernquist!,
        set_quadrupole!, set_kappa!, parallel_poisson, compute_gravitational_potential,
-       apply_kick_step!, apply_drift_step!, step!, reduce_diagnostics, cfl_limits
+       apply_kick_step!, apply_drift_step!, step!, reduce_diagnostics, cfl_limits, step_properties,
+       accelerations
 
 const libwdm = get(ENV, "WAVEDM_HIP_LIB", joinpath(@__DIR__, "..", "lib", "libwavedm_hip.so"))
 
@@ -37,6 +38,7 @@ const WDM_V_HERNQUIST = Cint(2)
 const WDM_V_QUADRUPOLE = Cint(4)
 const WDM_V_KAPPA = Cint(8)
 const WDM_RED_COUNT = 6
+const WDM_DIAG_COUNT = 22
 const WDM_ADJ_DT_SCALE = Cuint(1)
 const WDM_ADJ_NPARAM = 6
 
@@ -197,6 +199,31 @@ function reduce_diagnostics(p::HIPPlan)
     out = zeros(Float64, WDM_RED_COUNT)
     check(ccall((:wdm_reduce, libwdm), Cint, (Ptr{Cvoid}, Ptr{Cdouble}, Cint), p.h, out, WDM_RED_COUNT), p.h)
     return (mass=out[1], kinetic=out[2], w_self=out[3], v_ext=out[4], psi_max=out[5], phi_max=out[6])
+end
+
+"""
+    step_properties(p::HIPPlan) -> NamedTuple
+
+The loop's per-step diagnostics of the plan's ψ, evaluated on the device
+(src/simulation_main.jl:550-558,641-656; src/visualization/runtime.jl:210-238),
+under the Prop.csv column names of src/io/save.jl:46-67 (lengths in code
+units: multiply by uL as the loop does).  `peak` is the 1-based
+CartesianIndex of findmax(ρ).
+"""
+function step_properties(p::HIPPlan)
+    o = zeros(Float64, WDM_DIAG_COUNT)
+    check(ccall((:wdm_diagnostics, libwdm), Cint, (Ptr{Cvoid}, Ptr{Cdouble}, Cint), p.h, o, WDM_DIAG_COUNT), p.h)
+    return (TotalMass=o[1], rho_max=o[2], peak=CartesianIndex(Int(o[3]) + 1, Int(o[4]) + 1, Int(o[5]) + 1),
+            R=o[6], R1=o[7], R2=o[8], R3=o[9], R4=o[10], R5=o[11], R6=o[12], R7=o[13], R8=o[14], R9=o[15],
+            MomentumX=o[16], MomentumY=o[17], MomentumZ=o[18], PE_abs=o[19], KE=o[20], QE=o[21], Virial=o[22])
+end
+
+"ax_all, ay_all, az_all = grad_central(-Δ..., Φ_all) of the plan's ψ (src/simulation_main.jl:652)."
+function accelerations(p::HIPPlan)
+    ax, ay, az = (Array{Float64}(undef, p.dims) for _ in 1:3)
+    check(ccall((:wdm_acceleration, libwdm), Cint, (Ptr{Cvoid}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}),
+                p.h, ax, ay, az), p.h)
+    return ax, ay, az
 end
 
 "(dt_phase, dt_grav, dt_SI) of src/physics/poisson.jl:35-50 on the plan's ψ."

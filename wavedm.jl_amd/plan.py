@@ -191,6 +191,32 @@ class Plan:
         keys = ["mass", "kinetic", "w_self", "v_ext", "psi_max", "phi_max"]
         return {k: float(out[i]) for i, k in enumerate(keys[:n])}
 
+    # Column names of the reference's Prop.csv (src/io/save.jl:46-67) for the
+    # WDM_DIAG_* slots, in slot order.
+    DIAG_KEYS = ("TotalMass", "rho_max", "peak_ix", "peak_iy", "peak_iz", "R",
+                 "R1", "R2", "R3", "R4", "R5", "R6", "R7", "R8", "R9",
+                 "MomentumX", "MomentumY", "MomentumZ", "PE_abs", "KE", "QE", "Virial")
+
+    def diagnostics(self, n=A.WDM_DIAG_COUNT):
+        """The reference loop's per-step diagnostics of the current psi
+        (src/simulation_main.jl:550-558,641-656; src/visualization/runtime.jl:210-238),
+        keyed by the Prop.csv column names; ``n`` < WDM_DIAG_PE_ABS + 1 skips
+        the potential."""
+        out = np.zeros(A.WDM_DIAG_COUNT, dtype=np.float64)
+        self._chk(self.lib.wdm_diagnostics(self._h, _dptr(out), int(n)))
+        d = {k: float(out[i]) for i, k in enumerate(self.DIAG_KEYS[:n])}
+        for k in ("peak_ix", "peak_iy", "peak_iz"):
+            if k in d:
+                d[k] = int(d[k])
+        return d
+
+    def acceleration(self):
+        """-grad Phi_all by central differences (grad_central(-dx, Phi_all),
+        src/simulation_main.jl:652): three real arrays of the grid shape."""
+        out = [np.empty(self.ncell, dtype=np.float64) for _ in range(3)]
+        self._chk(self.lib.wdm_acceleration(self._h, *[_dptr(a) for a in out]))
+        return tuple(a.reshape(self.shape, order="F") for a in out)
+
     def cfl(self, kappa=0.0):
         out = np.zeros(3, dtype=np.float64)
         self._chk(self.lib.wdm_cfl(self._h, float(kappa), _dptr(out)))
