@@ -16,6 +16,10 @@
 //                w_k = (hbar/m) k^2 dt / 2        (sigma scales dt and dt_kick)
 // Memory: checkpoints of psi every K steps; each segment is recomputed with the
 // fused forward step while psi_n and Phi_n are taped, then walked backwards.
+// A backward step is eight passes like a forward step: the y/z passes of the
+// conjugate drift, k_adj_xs (x inverse, s, partials, R2C of s), the Poisson
+// y/z passes, k_adj_xg (C2R of u, g_n, absorber, x forward); see
+// wdm_adj_kernels.hpp.
 // No counterpart in the reference (it imports Zygote, src/WaveDM.jl:42, but
 // defines no rule for the step).
 
@@ -59,47 +63,10 @@ WDM_FI void block_sums(double (&acc)[NV], double (*sh)[256], double* __restrict_
         for (int j = 0; j < NV; ++j) partials[NV * blockIdx.x + j] = sh[j][0];
 }
 
-constexpr int kAdjNV = 6;
+constexpr int kAdjNV = kAdjParams;
 
-// s = tau Im(conj(g') E psi) and the parameter partial sums.
-template <class T>
-WDM_KERNEL(256)
-void k_adj_s(const cx<T>* __restrict__ g, const cx<T>* __restrict__ psi, const double* __restrict__ phi,
-             double* __restrict__ s, int nx, int ny, int nz, const double* __restrict__ xs,
-             const double* __restrict__ ys, const double* __restrict__ zs, VModel vm, double tau,
-             double* __restrict__ partials) {
-    WDM_SHARED double sh[kAdjNV][256];
-    double acc[kAdjNV] = {0, 0, 0, 0, 0, 0};
-    WDM_FOR_CELLS(nx, ny, nz) {
-        const cx<T> pv = psi[i];
-        const double V = vm.flags ? eval_V<T>(vm, xs[x], ys[y], zs[z], i, pv) : 0.0;
-        const double ph = (phi ? phi[i] : 0.0) + V;
-        const cx<double> e = expmi(tau * ph);
-        const double pr = double(pv.x), pi = double(pv.y);
-        const double qr = e.x * pr - e.y * pi, qi = e.x * pi + e.y * pr;  // psi'
-        const double gr = double(g[i].x), gi = double(g[i].y);
-        const double sv = tau * (gr * qi - gi * qr);
-        s[i] = sv;
-        if (vm.flags & V_HERNQUIST) {
-            const double dx = xs[x] - vm.hq_c[0], dy = ys[y] - vm.hq_c[1], dz = zs[z] - vm.hq_c[2];
-            const double r = __builtin_sqrt(dx * dx + dy * dy + dz * dz);
-            const double ra = r + vm.hq_a;
-            const double G = vm.hq_G;
-            acc[0] += sv * (-G / ra);
-            const double dVdr = vm.hq_GM / (ra * ra);
-            acc[1] += sv * dVdr;
-            if (r > 0.0) {
-                acc[2] -= sv * dVdr * dx / r;
-                acc[3] -= sv * dVdr * dy / r;
-                acc[4] -= sv * dVdr * dz / r;
-            }
-        }
-        acc[5] += sv * ph;
-    }
-    block_sums<kAdjNV>(acc, sh, partials);
-}
-
-// g = conj(E) g' + 2 (u + kappa s) psi  (then * border along x when given)
+// g = conj(E) g' + 2 (u + kappa s) psi  (then * border along x when given);
+// the element-wise form of k_adj_xg, used for B g_N before the sweep
 template <class T>
 WDM_KERNEL(256)
 void k_adj_g(cx<T>* __restrict__ g, const cx<T>* __restrict__ psi, const double* __restrict__ phi,
@@ -182,7 +149,7 @@ struct AdjState {
     DBuf tpsi;   // K psi of the current segment
     DBuf tphi;   // K real Phi (FP64) of the current segment
     DBuf tspec;  // K 3-D spectra F(psi') of the current segment (dt-scale term)
-    DBuf g, s, u;  // adjoint field and real fields
+    DBuf g;      // adjoint field
     DBuf part, acc;
     DBuf kfull[3];
 };
@@ -190,7 +157,7 @@ struct AdjState {
 void adj_release(wdm_plan* p) {
     if (!p->adj) return;
     AdjState& A = *p->adj;
-    for (DBuf* b : {&A.ckpt, &A.tpsi, &A.tphi, &A.tspec, &A.g, &A.s, &A.u, &A.part, &A.acc, &A.kfull[0], &A.kfull[1],
+    for (DBuf* b : {&A.ckpt, &A.tpsi, &A.tphi, &A.tspec, &A.g, &A.part, &A.acc, &A.kfull[0], &A.kfull[1],
                     &A.kfull[2]})
         b->reset();
     delete p->adj;
@@ -231,18 +198,19 @@ void adj_forward(wdm_plan* p, int nsteps, double dt, double dt_kick, double t0, 
     A.ready = true;
 }
 
-// g (device) <- U^H g, the conjugate drift; with Psi given, also the block
-// partials of the dt-scale spectral term into part (unfused passes)
+// g (device, transformed along x) <- the y and z passes of the conjugate drift
+// U^H g (the x passes are fused into k_adj_xg / k_adj_xs); with Psi given,
+// also the block partials of the dt-scale spectral term into part (unfused
+// z passes)
 template <class T>
-void adj_conj_drift(wdm_plan* p, cx<T>* g, const cx<T>* Psi, double* part) {
+void adj_conj_drift_yz(wdm_plan* p, cx<T>* g, const cx<T>* Psi, double* part) {
     typedef typename TwEntry<T>::type TwE;
     AdjState& A = adj_of(p);
     const AxisOps<T>* ox = ops_of<T>(p, 0);
     const AxisOps<T>* oy = ops_of<T>(p, 1);
     const AxisOps<T>* oz = ops_of<T>(p, 2);
     KTabs none = {nullptr, nullptr, nullptr, 0.0, nullptr};
-    const int nlines = p->ny * p->nz;
-    ox->x_fft(-1, g, nlines, p->ax[0].tw.as<TwE>(), p->stream);
+    (void)ox;
     if (p->ny > 1)
         oy->col(COL_FWD, KF_NONE, g, p->nx, p->nz, unsigned(p->nx), unsigned(p->nx * p->ny), p->ax[1].tw.as<TwE>(),
                 none, p->stream);
@@ -267,7 +235,6 @@ void adj_conj_drift(wdm_plan* p, cx<T>* g, const cx<T>* Psi, double* part) {
     if (p->ny > 1)
         oy->col(COL_INV, KF_NONE, g, p->nx, p->nz, unsigned(p->nx), unsigned(p->nx * p->ny), p->ax[1].tw.as<TwE>(),
                 none, p->stream);
-    ox->x_fft(+1, g, nlines, p->ax[0].tw.as<TwE>(), p->stream);
     hip_check(hipGetLastError(), "conjugate drift");
 }
 
@@ -288,13 +255,13 @@ void adj_backward(wdm_plan* p, const void* grad, int dtype, void* grad0, double*
     cx<T>* tpsi = A.tpsi.get<cx<T>>(n * size_t(A.K));
     double* tphi = p->self_gravity ? A.tphi.get<double>(n * size_t(A.K)) : nullptr;
     cx<T>* g = A.g.get<cx<T>>(n);
-    double* s = A.s.get<double>(n);
-    double* u = p->self_gravity ? A.u.get<double>(n) : nullptr;
-    // block partials of every step, summed once at the end in a fixed order
+    // workgroup partials of every step, summed once at the end in a fixed order
     // (the result does not depend on the checkpoint interval):
-    // [nsteps][kAdjNV][nb] parameter sums, then [nsteps][nb] dt-scale spectral sums
-    double* part = A.part.get<double>(size_t(A.nsteps) * (kAdjNV + 1) * nb);
-    double* part_spec = part + size_t(A.nsteps) * kAdjNV * nb;
+    // [nsteps][nbx][kAdjNV] parameter sums of k_adj_xs, then [nsteps][nb]
+    // dt-scale spectral sums of k_adj_spec
+    const unsigned nbx = unsigned(((nlines + ox->lpw - 1) / ox->lpw + ox->nw - 1) / ox->nw);
+    double* part = A.part.get<double>(size_t(A.nsteps) * (size_t(kAdjNV) * nbx + nb));
+    double* part_spec = part + size_t(A.nsteps) * kAdjNV * nbx;
     double* acc = A.acc.get<double>(kAdjNV);
     cx<T>* tspec = want_dt ? A.tspec.get<cx<T>>(n * size_t(A.K)) : nullptr;
     if (want_dt) {
@@ -321,9 +288,10 @@ void adj_backward(wdm_plan* p, const void* grad, int dtype, void* grad0, double*
         VModel none;
         std::memset(&none, 0, sizeof(none));
         WDM_LAUNCH((k_adj_g<T>), dim3(nb), dim3(256), p->stream, g, (const cx<T>*)p->psi.as<cx<T>>(),
-                   (const double*)nullptr, (const double*)nullptr, (const double*)s, p->nx, p->ny, p->nz, xs, ys, zs,
-                   none, 0.0, border);
+                   (const double*)nullptr, (const double*)nullptr, (const double*)nullptr, p->nx, p->ny, p->nz, xs,
+                   ys, zs, none, 0.0, border);
     }
+    ox->x_fft(-1, g, nlines, p->ax[0].tw.as<TwE>(), p->stream);  // the backward steps take g along x transformed
     const double tau = A.dt_kick / p->hbar_over_m;
     for (int k = A.nckpt - 1; k >= 0; --k) {
         const int n0 = k * A.K;
@@ -356,22 +324,27 @@ void adj_backward(wdm_plan* p, const void* grad, int dtype, void* grad0, double*
             const int step = n0 + i;
             const cx<T>* pn = tpsi + n * size_t(i);
             const double* fn = tphi ? tphi + n * size_t(i) : nullptr;
-            const VModel vm = make_vmodel(p, A.t0 + double(step) * A.dt);
-            adj_conj_drift<T>(p, g, tspec ? tspec + n * size_t(i) : nullptr, part_spec + size_t(step) * nb);
-            WDM_LAUNCH((k_adj_s<T>), dim3(nb), dim3(256), p->stream, (const cx<T>*)g, pn, fn, s, p->nx, p->ny, p->nz,
-                       xs, ys, zs, vm, tau, part + size_t(step) * kAdjNV * nb);
-            if (u) {
-                ox->x_r2c_real(s, p->rhohat.as<cx<T>>(), nlines, p->hpitch, p->ax[0].tw.as<TwE>(), p->stream);
-                poisson_chain<T>(p, p->four_pi_G);
-                ox->x_c2r_real(p->rhohat.as<cx<T>>(), u, nlines, p->hpitch, p->ax[0].tw.as<TwE>(), p->stream);
-            }
-            WDM_LAUNCH((k_adj_g<T>), dim3(nb), dim3(256), p->stream, g, pn, fn, (const double*)u, (const double*)s,
-                       p->nx, p->ny, p->nz, xs, ys, zs, vm, tau, step > 0 ? border : (const double*)nullptr);
+            AdjXArgs xa;
+            xa.tau = tau;
+            xa.Ny = p->ny;
+            xa.xs = xs;
+            xa.ys = ys;
+            xa.zs = zs;
+            xa.phi = fn;
+            xa.border = nullptr;
+            xa.vm = make_vmodel(p, A.t0 + double(step) * A.dt);
+            adj_conj_drift_yz<T>(p, g, tspec ? tspec + n * size_t(i) : nullptr, part_spec + size_t(step) * nb);
+            cx<T>* rh = p->self_gravity ? p->rhohat.as<cx<T>>() : nullptr;
+            ox->adj_xs(g, pn, rh, nlines, p->hpitch, p->ax[0].tw.as<TwE>(), xa,
+                       part + size_t(step) * kAdjNV * nbx, p->stream);
+            if (rh) poisson_chain<T>(p, p->four_pi_G);
+            xa.border = step > 0 ? border : nullptr;
+            ox->adj_xg(step > 0, g, pn, rh, nlines, p->hpitch, p->ax[0].tw.as<TwE>(), xa, p->stream);
             hip_check(hipGetLastError(), "adjoint step");
         }
     }
-    WDM_LAUNCH((k_adj_accumulate), dim3(kAdjNV), dim3(256), p->stream, (const double*)part, int(A.nsteps * nb),
-               kAdjNV, acc);
+    WDM_LAUNCH((k_adj_accumulate), dim3(kAdjNV), dim3(256), p->stream, (const double*)part,
+               int(size_t(A.nsteps) * nbx), kAdjNV, acc);
     if (want_dt)
         WDM_LAUNCH((k_adj_accumulate), dim3(1), dim3(256), p->stream, (const double*)part_spec, int(A.nsteps * nb), 1,
                    acc + 5);

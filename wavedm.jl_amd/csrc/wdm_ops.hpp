@@ -5,6 +5,7 @@
 #pragma once
 
 #include "wdm_kernels.hpp"
+#include "wdm_adj_kernels.hpp"
 
 #ifdef WDM_GPU
 namespace wdm {
@@ -87,6 +88,12 @@ struct AxisOps {
     void (*x_r2c_real)(const double* src, cx<T>* rhohat, int nlines, int hpitch, const TwE* tw, hipStream_t s);
     void (*x_c2r_real)(const cx<T>* h, double* dst, int nlines, int hpitch, const TwE* tw, hipStream_t s);
     void (*x_fft)(int dir, cx<T>* psi, int nlines, const TwE* tw, hipStream_t s);
+    // adjoint step x passes (k_adj_xs, k_adj_xg); adj_xs writes kAdjParams
+    // partials per workgroup, x_blocks of them
+    void (*adj_xs)(cx<T>* g, const cx<T>* psi_n, cx<T>* shat, int nlines, int hpitch, const TwE* tw,
+                   const AdjXArgs& a, double* partials, hipStream_t s);
+    void (*adj_xg)(bool fwd, cx<T>* g, const cx<T>* psi_n, const cx<T>* uhat, int nlines, int hpitch, const TwE* tw,
+                   const AdjXArgs& a, hipStream_t s);
 };
 
 // Grid sizes (in workgroups of NW waves).
@@ -224,9 +231,23 @@ struct Launch {
             WDM_LAUNCH((k_x_fft<T, PsiP, F, +1, NW>), dim3(x_blocks<NW, F>(nlines)), dim3(NW * 64), s, psi, nlines,
                        tw);
     }
+    static void adj_xs(cx<T>* g, const cx<T>* pn, cx<T>* sh, int nlines, int hp, const TwE* tw, const AdjXArgs& a,
+                       double* partials, hipStream_t s) {
+        WDM_LAUNCH((k_adj_xs<T, PsiP, F, NW>), dim3(x_blocks<NW, F>(nlines)), dim3(NW * 64), s, g, pn, sh, nlines, hp,
+                   tw, a, partials);
+    }
+    static void adj_xg(bool fwd, cx<T>* g, const cx<T>* pn, const cx<T>* uh, int nlines, int hp, const TwE* tw,
+                       const AdjXArgs& a, hipStream_t s) {
+        if (fwd)
+            WDM_LAUNCH((k_adj_xg<T, PsiP, F, true, NW>), dim3(x_blocks<NW, F>(nlines)), dim3(NW * 64), s, g, pn, uh,
+                       nlines, hp, tw, a);
+        else
+            WDM_LAUNCH((k_adj_xg<T, PsiP, F, false, NW>), dim3(x_blocks<NW, F>(nlines)), dim3(NW * 64), s, g, pn, uh,
+                       nlines, hp, tw, a);
+    }
     static const AxisOps<T>* table() {
         static const AxisOps<T> ops = {F::N, F::N1, F::N2, F::LPW, NW, &col, &x_kick, &x_inv_dens,
-                                       &x_dens, &x_r2c_real, &x_c2r_real, &x_fft};
+                                       &x_dens, &x_r2c_real, &x_c2r_real, &x_fft, &adj_xs, &adj_xg};
         return &ops;
     }
 };
