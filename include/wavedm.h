@@ -37,6 +37,7 @@
  *   wdm_diagnostics      the loop's per-step diagnostics   src/simulation_main.jl:550-558,641-656,
  *                        (Prop.csv columns)                src/visualization/runtime.jl:210-238
  *   wdm_acceleration     grad_central(-dx, Phi_all)        src/simulation_main.jl:652
+ *   wdm_density_*        time-averaged density buffer      src/simulation_main.jl:657-660
  */
 #ifndef WAVEDM_H
 #define WAVEDM_H
@@ -230,6 +231,16 @@ int wdm_diagnostics(wdm_plan* plan, double* out, int nout);
    src/simulation_main.jl:652; periodic neighbours on the faces) into three
    host arrays of nx*ny*nz doubles, x-fastest. */
 int wdm_acceleration(wdm_plan* plan, double* host_ax, double* host_ay, double* host_az);
+
+/* Time-averaged density on the device: the loop's buffer_psi2 += rho and
+   averaged_psi2 = buffer_psi2 / average_N (src/simulation_main.jl:657-660,
+   src/io/save.jl:79-81).  wdm_density_accumulate adds |psi|^2 of the current
+   psi to an FP64 device sum; wdm_density_average writes sum / count to a host
+   array of nx*ny*nz doubles (host_out may be NULL to read only *count);
+   wdm_density_reset clears the sum. */
+int wdm_density_accumulate(wdm_plan* plan);
+int wdm_density_average(wdm_plan* plan, double* host_out, long long* count);
+int wdm_density_reset(wdm_plan* plan);
 
 /* Batched ensemble of independent plans (uneven grids; no counterpart in the
    reference, which steps one box at a time).  The plans stay owned by the

@@ -8,9 +8,9 @@ src/parallel_poisson.jl, src/physics/poisson.jl, src/solver/fft_operators.jl)
 over that ABI, used by the tests, the benchmark and Python callers.  The
 directory name is not an identifier, so it is imported as ``wavedm_amd``.
 """
-from . import _capi, adjoint, configs, ensemble, ic, kdk, units
+from . import _capi, adjoint, configs, ensemble, ic, kdk, snapshot, units
 from ._capi import WaveDMError
 from .ensemble import Ensemble
 from .plan import Plan
 
-__all__ = ["Ensemble", "Plan", "WaveDMError", "_capi", "adjoint", "configs", "ensemble", "ic", "kdk", "units"]
+__all__ = ["Ensemble", "Plan", "WaveDMError", "_capi", "adjoint", "configs", "ensemble", "ic", "kdk", "snapshot", "units"]

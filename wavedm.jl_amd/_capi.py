@@ -85,6 +85,9 @@ SIGNATURES = {
     "wdm_reduce": (_I, [_P, _DP, _I]),
     "wdm_diagnostics": (_I, [_P, _DP, _I]),
     "wdm_acceleration": (_I, [_P, _DP, _DP, _DP]),
+    "wdm_density_accumulate": (_I, [_P]),
+    "wdm_density_average": (_I, [_P, _DP, C.POINTER(C.c_longlong)]),
+    "wdm_density_reset": (_I, [_P]),
     "wdm_cfl": (_I, [_P, _D, _DP]),
     "wdm_synchronize": (_I, [_P]),
     "wdm_step_timed": (_I, [_P, _I, _D, _D, _D, _DP, _I, _DP]),

@@ -194,6 +194,15 @@ void k_diag_hist_sum(const unsigned long long* __restrict__ hpart, int nrows, in
     out[size_t(kDiagBins) * blockIdx.y + bin] = s;
 }
 
+WDM_KERNEL(256)
+void k_fill_zero(double* __restrict__ a, int nx, int ny, int nz) {
+    WDM_FOR_CELLS(nx, ny, nz) {
+        (void)y;
+        (void)z;
+        a[i] = 0.0;
+    }
+}
+
 // phi += V_ext (static, Hernquist, quadrupole; the kappa term is not part of Phi_all)
 template <class T>
 WDM_KERNEL(256)
@@ -379,9 +388,63 @@ void acceleration(wdm_plan* p, double* const host[3]) {
     }
 }
 
+// acc += |psi|^2 (FP64), the loop's time-averaged density buffer
+template <class T>
+WDM_KERNEL(256)
+void k_density_accumulate(const cx<T>* __restrict__ psi, double* __restrict__ acc, int nx, int ny, int nz) {
+    WDM_FOR_CELLS(nx, ny, nz) {
+        (void)y;
+        (void)z;
+        acc[i] += rho_of(psi[i]);
+    }
+}
+
+template <class T>
+void density_accumulate(wdm_plan* p) {
+    if (p->kspace) fail(WDM_ERR_STATE, "wdm_density_accumulate: psi is in k-space");
+    double* acc = p->rho_avg.as<double>();
+    if (p->rho_avg_count == 0) {
+        acc = p->rho_avg.get<double>(p->ncell);
+        WDM_LAUNCH((k_fill_zero), dim3(adj_line_blocks(p->ny, p->nz)), dim3(256), p->stream, acc, p->nx, p->ny, p->nz);
+    }
+    WDM_LAUNCH((k_density_accumulate<T>), dim3(adj_line_blocks(p->ny, p->nz)), dim3(256), p->stream,
+               (const cx<T>*)p->psi.as<cx<T>>(), acc, p->nx, p->ny, p->nz);
+    hip_check(hipGetLastError(), "density accumulate");
+    hip_check(hipStreamSynchronize(p->stream), "density accumulate sync");
+    ++p->rho_avg_count;
+}
+
 }  // namespace
 
 extern "C" {
+
+int wdm_density_accumulate(wdm_plan* p) {
+    if (!p) return fail_code(nullptr, WDM_ERR_ARG, "plan is NULL");
+    return guarded(p, [&] { DISPATCH(p, density_accumulate, p); });
+}
+
+int wdm_density_average(wdm_plan* p, double* host_out, long long* count) {
+    if (!p) return fail_code(nullptr, WDM_ERR_ARG, "plan is NULL");
+    return guarded(p, [&] {
+        if (count) *count = p->rho_avg_count;
+        if (!host_out) return;
+        if (p->rho_avg_count == 0) fail(WDM_ERR_STATE, "wdm_density_average: nothing accumulated");
+        std::vector<double> h(p->ncell);
+        hip_check(hipMemcpyAsync(h.data(), p->rho_avg.as<double>(), sizeof(double) * p->ncell, hipMemcpyDeviceToHost,
+                                 p->stream),
+                  "density average copy");
+        hip_check(hipStreamSynchronize(p->stream), "density average sync");
+        const double inv = 1.0 / double(p->rho_avg_count);
+        for (size_t i = 0; i < p->ncell; ++i) host_out[i] = h[i] * inv;
+    });
+}
+
+int wdm_density_reset(wdm_plan* p) {
+    if (!p) return fail_code(nullptr, WDM_ERR_ARG, "plan is NULL");
+    p->rho_avg_count = 0;
+    p->rho_avg.reset();
+    return WDM_OK;
+}
 
 int wdm_diagnostics(wdm_plan* p, double* out, int nout) {
     if (!p || !out || nout <= 0) return fail_code(p, WDM_ERR_ARG, "wdm_diagnostics: bad argument");

@@ -128,6 +128,8 @@ struct wdm_plan {
     AxisState ax[3];
     DBuf psi, rhohat, partials, redout, border, vstatic, scratch, scratch_real;
     DBuf diag;  // diagnostics partials and histograms
+    DBuf rho_avg;  // sum of |psi|^2 over wdm_density_accumulate calls (FP64)
+    long long rho_avg_count = 0;
     size_t npartials = 0;
     bool border_on = false;
     // state
@@ -1016,7 +1018,7 @@ int wdm_plan_destroy(wdm_plan* p) {
         a.k2.reset();
     }
     for (DBuf* b : {&p->psi, &p->rhohat, &p->partials, &p->redout, &p->border, &p->vstatic, &p->scratch,
-                    &p->scratch_real, &p->diag})
+                    &p->scratch_real, &p->diag, &p->rho_avg})
         b->reset();
     if (p->stream) hipStreamDestroy(p->stream);
     delete p;

@@ -23,7 +23,7 @@ using ChainRulesCore
 export HIPPlan, ssfm, set_psi!, get_psi, set_border!, set_static_potential!, set_hernquist!,
        set_quadrupole!, set_kappa!, parallel_poisson, compute_gravitational_potential,
        apply_kick_step!, apply_drift_step!, step!, reduce_diagnostics, cfl_limits, step_properties,
-       accelerations
+       accelerations, density_accumulate!, density_average, density_reset!
 
 const libwdm = get(ENV, "WAVEDM_HIP_LIB", joinpath(@__DIR__, "..", "lib", "libwavedm_hip.so"))
 
@@ -225,6 +225,21 @@ function accelerations(p::HIPPlan)
                 p.h, ax, ay, az), p.h)
     return ax, ay, az
 end
+
+"buffer_ψ2 += ρ on the device (src/simulation_main.jl:659-660)."
+density_accumulate!(p::HIPPlan) = (check(ccall((:wdm_density_accumulate, libwdm), Cint, (Ptr{Cvoid},), p.h), p.h); p)
+
+"(averaged_ψ2, average_N) of the accumulated steps (src/io/save.jl:79-81)."
+function density_average(p::HIPPlan)
+    n = Ref{Clonglong}(0)
+    check(ccall((:wdm_density_average, libwdm), Cint, (Ptr{Cvoid}, Ptr{Cdouble}, Ref{Clonglong}), p.h, C_NULL, n), p.h)
+    n[] == 0 && return (nothing, 0)
+    avg = Array{Float64}(undef, p.dims)
+    check(ccall((:wdm_density_average, libwdm), Cint, (Ptr{Cvoid}, Ptr{Cdouble}, Ref{Clonglong}), p.h, avg, n), p.h)
+    return avg, Int(n[])
+end
+
+density_reset!(p::HIPPlan) = (check(ccall((:wdm_density_reset, libwdm), Cint, (Ptr{Cvoid},), p.h), p.h); p)
 
 "(dt_phase, dt_grav, dt_SI) of src/physics/poisson.jl:35-50 on the plan's ψ."
 function cfl_limits(p::HIPPlan; κ::Real=0.0)

@@ -71,6 +71,7 @@ class Plan:
         self._chk(self.lib.wdm_set_coeffs(self._h, self.hbar_over_m, self.four_pi_G))
 
     def set_self_gravity(self, on: bool):
+        self.self_gravity = bool(on)
         self._chk(self.lib.wdm_set_self_gravity(self._h, 1 if on else 0))
 
     def set_border(self, border):
@@ -216,6 +217,24 @@ class Plan:
         out = [np.empty(self.ncell, dtype=np.float64) for _ in range(3)]
         self._chk(self.lib.wdm_acceleration(self._h, *[_dptr(a) for a in out]))
         return tuple(a.reshape(self.shape, order="F") for a in out)
+
+    def density_accumulate(self):
+        """buffer_psi2 += |psi|^2 on the device (src/simulation_main.jl:659-660)."""
+        self._chk(self.lib.wdm_density_accumulate(self._h))
+
+    def density_average(self):
+        """(buffer / count, count): the time-averaged density of the accumulated
+        steps (src/io/save.jl:79-81), or (None, 0) when nothing was accumulated."""
+        n = C.c_longlong(0)
+        self._chk(self.lib.wdm_density_average(self._h, None, C.byref(n)))
+        if n.value == 0:
+            return None, 0
+        out = np.empty(self.ncell, dtype=np.float64)
+        self._chk(self.lib.wdm_density_average(self._h, _dptr(out), C.byref(n)))
+        return out.reshape(self.shape, order="F"), int(n.value)
+
+    def density_reset(self):
+        self._chk(self.lib.wdm_density_reset(self._h))
 
     def cfl(self, kappa=0.0):
         out = np.zeros(3, dtype=np.float64)
