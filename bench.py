@@ -79,7 +79,7 @@ def energy(red):
     return red["kinetic"] + red["w_self"] + red["v_ext"]
 
 
-def run_gpu(W, cfg, precision, steps, warmup):
+def run_gpu(W, cfg, precision, steps, warmup, ledger=False, diagnostics=True):
     C = W.configs
     with W.Plan(cfg["shape"], cfg["xmax"], precision, hbar_over_m=cfg["hbar_over_m"],
                 four_pi_G=cfg["four_pi_G"]) as p:
@@ -88,6 +88,7 @@ def run_gpu(W, cfg, precision, steps, warmup):
         if warmup:
             p.step(warmup, dt)
             C.install(p, cfg)
+        p.set_mass_ledger(ledger)
         red0 = p.reduce()
         p.synchronize()
         t0 = time.perf_counter()
@@ -95,6 +96,8 @@ def run_gpu(W, cfg, precision, steps, warmup):
         p.synchronize()
         wall = time.perf_counter() - t0
         red1 = p.reduce()
+        if not diagnostics:
+            return dict(dt=dt, wall=wall, red0=red0, red1=red1)
         # the reference loop's per-step diagnostics on the final state (device)
         diag_ms = {}
         for label, nslots in (("full", W._capi.WDM_DIAG_COUNT), ("without_potential", W._capi.WDM_DIAG_PE_ABS)):
@@ -345,6 +348,12 @@ def main():
     m0, m1 = r32["red0"]["mass"], r32["red1"]["mass"]
     out["fp32"] = {"mass_drift": abs(m1 - m0) / m0, "energy0": energy(r32["red0"]),
                    "energy1": energy(r32["red1"])}
+    print("[bench] FP32 leg with the mass projection", file=sys.stderr, flush=True)
+    rl = run_gpu(W, cfg, "f32", args.steps, min(args.warmup, 5), ledger=True, diagnostics=False)
+    out["fp32"]["mass_projection"] = {
+        "ms_per_step": 1e3 * rl["wall"] / args.steps,
+        "mass_drift": abs(rl["red1"]["mass"] - rl["red0"]["mass"]) / rl["red0"]["mass"],
+        "note": "wdm_set_mass_ledger(1): scale from the x pass's mass partials applied in the next z pass"}
     out["diagnostics"] = {"ms_per_call": r32["diag_ms"], "values": r32["diag"],
                           "note": "wdm_diagnostics on the final FP32 state: mass, peak, R, R1..R9, momenta, "
                                   "PE_abs/KE/QE/Virial (full includes the Phi_all solve)"}

@@ -103,6 +103,17 @@ int wdm_plan_destroy(wdm_plan* plan);
 int wdm_set_coeffs(wdm_plan* plan, double hbar_over_m, double four_pi_G);
 int wdm_set_self_gravity(wdm_plan* plan, int on);
 
+/* Mass projection (no counterpart in the reference; opt-in, off by default):
+   after every step psi is scaled so that sum |psi|^2 keeps the value it had
+   when the projection was switched on or psi was last set, removing the
+   random walk of FP32 rounding in the norm.  The scale f of a step is
+   computed on the device from the inverse x pass's mass partials and applied
+   by the next step (its z pass multiplies the FP64 drift factor by f, its
+   kick uses Phi f^2), so the state a call returns carries the rounding of its
+   last step only, and the deviation stays at that level instead of growing.
+   Needs self-gravity and no absorber; not differentiated by the adjoint. */
+int wdm_set_mass_ledger(wdm_plan* plan, int on);
+
 int wdm_set_psi(wdm_plan* plan, const void* host_psi, int dtype);
 int wdm_get_psi(wdm_plan* plan, void* host_psi, int dtype);
 /* device-pointer variants (memory from hipMalloc / a torch tensor on this GPU) */

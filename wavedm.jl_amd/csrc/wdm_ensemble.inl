@@ -132,6 +132,7 @@ void ens_prepare(wdm_ensemble* e, const double* dt) {
     for (size_t i = 0; i < e->m.size(); ++i) {
         wdm_plan* p = e->m[i];
         if (p->kspace) fail(WDM_ERR_STATE, "wdm_ensemble_step: member " + std::to_string(i) + " is in k-space");
+        DISPATCH(p, ledger_begin, p);  // outside any capture: it reads the mass back
         DISPATCH(p, ensure_phi, p);
         ensure_drift(p, dt[i]);
     }

@@ -56,6 +56,7 @@ struct KTabs {
     double scale;
     void* spec_out;  // COL_FMIS: the forward spectrum (before the factor) is
                      // also stored here, same layout as data
+    const double* norm;  // KF_DRIFT: device scalar multiplying the factor (mass projection) or null
 };
 
 // line-axis table entry of a factor kind
@@ -93,8 +94,9 @@ WDM_FI KfLane<KF, T> kf_lane(const KTabs& kt, int c, int o) {
     if constexpr (KF == KF_DRIFT) {
         const cx<double> a = static_cast<const cx<double>*>(kt.col)[c];
         const cx<double> b = static_cast<const cx<double>*>(kt.oth)[o];
-        l.ar = (a.x * b.x - a.y * b.y) * kt.scale;
-        l.ai = (a.x * b.y + a.y * b.x) * kt.scale;
+        const double sc = kt.norm ? kt.scale * kt.norm[0] : kt.scale;
+        l.ar = (a.x * b.x - a.y * b.y) * sc;
+        l.ai = (a.x * b.y + a.y * b.x) * sc;
     } else if constexpr (KF == KF_POISSON) {
         l.kco = static_cast<const T*>(kt.col)[c] + static_cast<const T*>(kt.oth)[o];
         l.scale = T(kt.scale);
@@ -657,6 +659,7 @@ struct KickArgs {
     double x0, x1;       // x(i) = (1 - t) x0 + t x1, t = i * xinv (the LinRange formula)
     double xinv;
     VModel vm;
+    const double* norm;  // pending mass-projection scale (device scalar) or null
 };
 
 // Kick (+ optional forward x-FFT).  Reads psi rows in real space and Phi_hat
@@ -720,6 +723,10 @@ void k_x_kick(cx<T>* __restrict__ psi, const cx<T>* __restrict__ phihat, int nli
     WDM_SCHED_FENCE();
     WDM_UNROLL
     for (int n1 = 0; n1 < N1; ++n1) v[n1] = WDM_LDG(ok, psi, rbase + unsigned(N2 * n1 + m.j));
+    // mass projection: this step's z pass scales psi by f, so the kick uses
+    // the potential of the projected state, Phi f^2 (linear in |psi|^2)
+    const double nf = ka.norm ? ka.norm[0] : 1.0;
+    const double nf2 = nf * nf;
     WDM_UNROLL
     for (int n1 = 0; n1 < N1; ++n1) {
         if ((n1 & 3) == 0) WDM_SCHED_FENCE();
@@ -735,12 +742,14 @@ void k_x_kick(cx<T>* __restrict__ psi, const cx<T>* __restrict__ phihat, int nli
                 const double dx = xc - ka.vm.hq_c[0];
                 V = -ka.vm.hq_GM / (__builtin_sqrt(fmaT(dx, dx, lv.hr2)) + ka.vm.hq_a);
             } else {
-                if (ka.vm.flags) V = eval_V_line<T>(ka.vm, lv, xc, unsigned(lc) * N + x, v[n1]);
+                if (ka.vm.flags)
+                    V = eval_V_line<T>(ka.vm, lv, xc, unsigned(lc) * N + x, ka.norm ? scale_dyn<P>(v[n1], nf) : v[n1]);
             }
         }
         T ph = phi[n1];
         WDM_OPAQUE(ph);
-        v[n1] = kick_rot(v[n1], ka.mh_dt * (double(ph) + V));
+        const double phd = ka.norm ? double(ph) * nf2 : double(ph);
+        v[n1] = kick_rot(v[n1], ka.mh_dt * (phd + V));
     }
     if constexpr (FWD) {
         fwd2<F, -1, P>(v, X, m.j, xb, tw);

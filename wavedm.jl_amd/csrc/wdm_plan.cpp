@@ -130,6 +130,9 @@ struct wdm_plan {
     DBuf diag;  // diagnostics partials and histograms
     DBuf rho_avg;  // sum of |psi|^2 over wdm_density_accumulate calls (FP64)
     long long rho_avg_count = 0;
+    // opt-in mass projection (wdm_set_mass_ledger): [reference mass, pending scale]
+    bool ledger = false, ledger_stale = true;
+    DBuf ledger_buf;
     size_t npartials = 0;
     bool border_on = false;
     // state
@@ -381,8 +384,14 @@ KickArgs make_kick(const wdm_plan* p, double dt_kick, double t) {
         ka.xinv = p->nx > 1 ? 1.0 / double(p->nx - 1) : 0.0;
     }
     ka.vm = make_vmodel(p, t);
+    ka.norm = p->ledger ? p->ledger_buf.as<double>() + 1 : nullptr;
     return ka;
 }
+
+template <class T>
+void ledger_begin(wdm_plan* p);
+template <class T>
+void ledger_update(wdm_plan* p);
 
 template <class T>
 void ensure_phi(wdm_plan* p) {
@@ -426,7 +435,7 @@ void z_pass(wdm_plan* p, int mode, double dt) {
     if (mode == COL_FMI || mode == COL_MI) {
         ensure_drift(p, dt);
         kt = {p->ax[0].drift.p, p->ax[1].drift.p, p->ax[2].drift.p, 1.0 / double(p->ncell),
-              mode == COL_FMI ? p->zspec_out : nullptr};
+              mode == COL_FMI ? p->zspec_out : nullptr, p->ledger ? p->ledger_buf.as<double>() + 1 : nullptr};
         if (mode == COL_FMI && p->zspec_out) mode = COL_FMIS;
     } else if (p->nz == 1) {
         return;
@@ -455,11 +464,13 @@ void x_inv_close(wdm_plan* p) {
         poisson_chain<T>(p, p->four_pi_G);
         p->phi_valid = true;
     }
+    ledger_update<T>(p);
 }
 
 template <class T>
 void do_kick(wdm_plan* p, double dt_kick, double t) {
     if (p->kspace) fail(WDM_ERR_STATE, "wdm_kick: psi is in k-space (call wdm_drift first)");
+    ledger_begin<T>(p);
     ensure_phi<T>(p);
     x_kick<T>(p, true, dt_kick, t);
     y_pass<T>(p, COL_FWD);
@@ -479,6 +490,7 @@ template <class T>
 void do_step(wdm_plan* p, int nsteps, double dt, double dt_kick, double t0, int split) {
     if (p->kspace) fail(WDM_ERR_STATE, "wdm_step: psi is in k-space (call wdm_drift first)");
     if (nsteps <= 0) return;
+    ledger_begin<T>(p);
     ensure_phi<T>(p);
     ensure_drift(p, dt);
     if (split == WDM_SPLIT_REFERENCE) {
@@ -645,6 +657,68 @@ void psi_sums(wdm_plan* p, double out[2]) {
     finalize(p, int(nb), out);
 }
 
+// ---------------------------------------------------------------- mass projection
+// Opt-in (wdm_set_mass_ledger).  ledger[0] = the reference mass (the mass of
+// psi when the projection was switched on or psi was last set), ledger[1] =
+// the pending scale f = sqrt(M_ref / M) computed from the per-wave mass
+// partials of the inverse x pass.  The next step applies it where an FP64
+// factor multiplies every element before a single rounding, the drift factor
+// of the z pass (psi_k *= P f), so the correction survives FP32 rounding
+// (a relative change of f - 1 ~ 1e-9 applied to FP32 values directly would
+// round back to the same values); the kick before it uses Phi f^2 and
+// kappa f^2, the potential of the projected state.  The sum runs in a fixed
+// order (per-thread compensated, then pairwise).
+WDM_KERNEL(1024)
+void k_ledger_update(const double* __restrict__ partials, int n, double* __restrict__ ledger) {
+    WDM_SHARED double sh[1024];
+    const int t = int(threadIdx.x);
+    double s = 0.0, comp = 0.0;
+    for (int i = t; i < n; i += 1024) {
+        const double v = partials[2 * i];
+        const double tt = s + v;
+        if (__builtin_fabs(s) >= __builtin_fabs(v)) comp += (s - tt) + v;
+        else comp += (v - tt) + s;
+        s = tt;
+    }
+    sh[t] = s + comp;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {  // pairwise, fixed order
+        if (t < w) sh[t] += sh[t + w];
+        __syncthreads();
+    }
+    if (t == 0) {
+        const double M = sh[0], ref = ledger[0];
+        ledger[1] = (M > 0.0 && ref > 0.0) ? __builtin_sqrt(ref / M) : 1.0;
+    }
+}
+
+WDM_KERNEL(64)
+void k_ledger_set(double* __restrict__ ledger, double m_ref, int set_ref) {
+    if (threadIdx.x == 0) {
+        if (set_ref) ledger[0] = m_ref;
+        ledger[1] = 1.0;
+    }
+}
+
+template <class T>
+void ledger_begin(wdm_plan* p) {
+    if (!p->ledger || !p->ledger_stale) return;
+    if (p->kspace) fail(WDM_ERR_STATE, "mass projection: psi is in k-space");
+    double ms[2];
+    psi_sums<T>(p, ms);
+    WDM_LAUNCH((k_ledger_set), dim3(1), dim3(64), p->stream, p->ledger_buf.get<double>(2), ms[0], 1);
+    hip_check(hipGetLastError(), "ledger set");
+    p->ledger_stale = false;
+}
+
+template <class T>
+void ledger_update(wdm_plan* p) {
+    if (!p->ledger || !p->mass_valid) return;
+    WDM_LAUNCH((k_ledger_update), dim3(1), dim3(1024), p->stream, (const double*)p->partials.as<double>(),
+               int(p->npartials / 2), p->ledger_buf.as<double>());
+    hip_check(hipGetLastError(), "ledger update");
+}
+
 template <class T>
 void reduce_all(wdm_plan* p, double* out, int nout) {
     if (p->kspace) fail(WDM_ERR_STATE, "wdm_reduce: psi is in k-space");
@@ -756,6 +830,7 @@ void ic_solitons(wdm_plan* p, const double* params, int nsol) {
     p->kspace = false;
     p->phi_valid = false;
     p->mass_valid = false;
+    p->ledger_stale = true;
 }
 
 template <class TD, class TS>
@@ -794,6 +869,7 @@ void set_psi_dev(wdm_plan* p, const void* src, int dtype, bool host) {
     p->kspace = false;
     p->phi_valid = false;
     p->mass_valid = false;
+    p->ledger_stale = true;
 }
 
 template <class T>
@@ -1018,7 +1094,7 @@ int wdm_plan_destroy(wdm_plan* p) {
         a.k2.reset();
     }
     for (DBuf* b : {&p->psi, &p->rhohat, &p->partials, &p->redout, &p->border, &p->vstatic, &p->scratch,
-                    &p->scratch_real, &p->diag, &p->rho_avg})
+                    &p->scratch_real, &p->diag, &p->rho_avg, &p->ledger_buf})
         b->reset();
     if (p->stream) hipStreamDestroy(p->stream);
     delete p;
@@ -1039,8 +1115,25 @@ int wdm_set_coeffs(wdm_plan* p, double hbar_over_m, double four_pi_G) {
 int wdm_set_self_gravity(wdm_plan* p, int on) {
     if (p) ++p->gen;
     if (!p) return fail_code(nullptr, WDM_ERR_ARG, "plan is NULL");
+    if (!on && p->ledger)
+        return fail_code(p, WDM_ERR_UNSUPPORTED, "wdm_set_self_gravity: the mass projection needs self-gravity");
     p->self_gravity = on != 0;
     p->phi_valid = false;
+    return WDM_OK;
+}
+
+int wdm_set_mass_ledger(wdm_plan* p, int on) {
+    if (!p) return fail_code(nullptr, WDM_ERR_ARG, "plan is NULL");
+    ++p->gen;
+    if (on && (!p->self_gravity || p->border_on))
+        return fail_code(p, WDM_ERR_UNSUPPORTED,
+                         "wdm_set_mass_ledger: needs self-gravity (its density pass gives the mass) and no absorber");
+    if (on && !p->ledger) {
+        const int rc = guarded(p, [&] { p->ledger_buf.get<double>(2); });
+        if (rc != WDM_OK) return rc;
+    }
+    p->ledger = on != 0;
+    p->ledger_stale = true;
     return WDM_OK;
 }
 
@@ -1076,6 +1169,8 @@ int wdm_set_border(wdm_plan* p, const double* border, int rank) {
         return WDM_OK;
     }
     if (rank != 1) return fail_code(p, WDM_ERR_UNSUPPORTED, "wdm_set_border: only rank 1 (length nx) is supported");
+    if (p->ledger)
+        return fail_code(p, WDM_ERR_UNSUPPORTED, "wdm_set_border: the mass projection assumes no absorber");
     return guarded(p, [&] {
         hip_check(hipStreamSynchronize(p->stream), "border sync");
         upload(p->border, std::vector<double>(border, border + p->nx));

@@ -23,7 +23,7 @@ using ChainRulesCore
 export HIPPlan, ssfm, set_psi!, get_psi, set_border!, set_static_potential!, set_hernquist!,
        set_quadrupole!, set_kappa!, parallel_poisson, compute_gravitational_potential,
        apply_kick_step!, apply_drift_step!, step!, reduce_diagnostics, cfl_limits, step_properties,
-       accelerations, density_accumulate!, density_average, density_reset!
+       accelerations, density_accumulate!, density_average, density_reset!, set_mass_ledger!
 
 const libwdm = get(ENV, "WAVEDM_HIP_LIB", joinpath(@__DIR__, "..", "lib", "libwavedm_hip.so"))
 
@@ -225,6 +225,10 @@ function accelerations(p::HIPPlan)
                 p.h, ax, ay, az), p.h)
     return ax, ay, az
 end
+
+"Opt-in mass projection for FP32 plans (no counterpart in the reference)."
+set_mass_ledger!(p::HIPPlan, on::Bool) =
+    (check(ccall((:wdm_set_mass_ledger, libwdm), Cint, (Ptr{Cvoid}, Cint), p.h, on), p.h); p)
 
 "buffer_ψ2 += ρ on the device (src/simulation_main.jl:659-660)."
 density_accumulate!(p::HIPPlan) = (check(ccall((:wdm_density_accumulate, libwdm), Cint, (Ptr{Cvoid},), p.h), p.h); p)

@@ -74,6 +74,11 @@ class Plan:
         self.self_gravity = bool(on)
         self._chk(self.lib.wdm_set_self_gravity(self._h, 1 if on else 0))
 
+    def set_mass_ledger(self, on: bool):
+        """Opt-in mass projection after every step (FP32 norm drift removal;
+        needs self-gravity and no absorber)."""
+        self._chk(self.lib.wdm_set_mass_ledger(self._h, 1 if on else 0))
+
     def set_border(self, border):
         if border is None:
             self._chk(self.lib.wdm_set_border(self._h, None, 1))
