@@ -109,9 +109,10 @@ int wdm_set_self_gravity(wdm_plan* plan, int on);
    random walk of FP32 rounding in the norm.  The scale f of a step is
    computed on the device from the inverse x pass's mass partials and applied
    by the next step (its z pass multiplies the FP64 drift factor by f, its
-   kick uses Phi f^2), so the state a call returns carries the rounding of its
-   last step only, and the deviation stays at that level instead of growing.
-   Needs self-gravity and no absorber; not differentiated by the adjoint. */
+   kick uses Phi f^2); wdm_step and wdm_drift apply the last one to the state
+   they return with unbiased stochastic rounding.  The deviation does not
+   grow with the number of steps.  Needs self-gravity and no absorber; not
+   differentiated by the adjoint. */
 int wdm_set_mass_ledger(wdm_plan* plan, int on);
 
 int wdm_set_psi(wdm_plan* plan, const void* host_psi, int dtype);

@@ -392,6 +392,8 @@ template <class T>
 void ledger_begin(wdm_plan* p);
 template <class T>
 void ledger_update(wdm_plan* p);
+template <class T>
+void ledger_finish(wdm_plan* p);
 
 template <class T>
 void ensure_phi(wdm_plan* p) {
@@ -484,6 +486,7 @@ void do_drift(wdm_plan* p, double dt) {
     z_pass<T>(p, COL_MI, dt);
     y_pass<T>(p, COL_INV);
     x_inv_close<T>(p);
+    ledger_finish<T>(p);
 }
 
 template <class T>
@@ -501,6 +504,7 @@ void do_step(wdm_plan* p, int nsteps, double dt, double dt_kick, double t0, int 
             y_pass<T>(p, COL_INV);
             x_inv_close<T>(p);
         }
+        ledger_finish<T>(p);
     } else {
         typedef typename TwEntry<T>::type TwE;
         const AxisOps<T>* ox = ops_of<T>(p, 0);
@@ -515,6 +519,7 @@ void do_step(wdm_plan* p, int nsteps, double dt, double dt_kick, double t0, int 
             if (s + 1 < nsteps) x_kick<T>(p, true, dt, t);
             else x_kick<T>(p, false, 0.5 * dt, t);
         }
+        ledger_finish<T>(p);  // the closing half kick used Phi f^2; psi still needs f
     }
 }
 
@@ -664,9 +669,10 @@ void psi_sums(wdm_plan* p, double out[2]) {
 // partials of the inverse x pass.  The next step applies it where an FP64
 // factor multiplies every element before a single rounding, the drift factor
 // of the z pass (psi_k *= P f), so the correction survives FP32 rounding
-// (a relative change of f - 1 ~ 1e-9 applied to FP32 values directly would
-// round back to the same values); the kick before it uses Phi f^2 and
-// kappa f^2, the potential of the projected state.  The sum runs in a fixed
+// (a relative change of f - 1 ~ 1e-9 applied to FP32 values with ordinary
+// rounding would round back to the same values); the kick before it uses
+// Phi f^2 and kappa f^2, the potential of the projected state.  The last
+// scale of a call is applied to the returned state by k_ledger_apply.  The sum runs in a fixed
 // order (per-thread compensated, then pairwise).
 WDM_KERNEL(1024)
 void k_ledger_update(const double* __restrict__ partials, int n, double* __restrict__ ledger) {
@@ -690,6 +696,53 @@ void k_ledger_update(const double* __restrict__ partials, int n, double* __restr
         const double M = sh[0], ref = ledger[0];
         ledger[1] = (M > 0.0 && ref > 0.0) ? __builtin_sqrt(ref / M) : 1.0;
     }
+}
+
+// End of a step sequence: psi *= f with unbiased stochastic rounding (the
+// product is formed in FP64 and rounded to the neighbour above or below with
+// probability given by the remainder, from a hash of the element index), so a
+// correction far below one FP32 ulp still moves the mass to M_ref on average;
+// Phi_hat *= f^2 with ordinary rounding.  FP64 plans round to nearest.
+WDM_FI unsigned ledger_hash(unsigned x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+// the neighbouring float of a finite r toward +inf (up) or -inf
+WDM_FI float ledger_next(float r, bool up) {
+    unsigned b = __builtin_bit_cast(unsigned, r);
+    if ((b & 0x7fffffffu) == 0u) b = up ? 1u : 0x80000001u;
+    else if ((b >> 31) == 0u) b = up ? b + 1u : b - 1u;
+    else b = up ? b - 1u : b + 1u;
+    return __builtin_bit_cast(float, b);
+}
+WDM_FI float ledger_round(double y, unsigned key) {
+    const float r = float(y);
+    const double d = y - double(r);
+    if (d == 0.0) return r;
+    const float nb = ledger_next(r, d > 0.0);
+    const double p = d / (double(nb) - double(r));  // in (0, 1/2]
+    const double u = double(ledger_hash(key) >> 8) * (1.0 / 16777216.0);
+    return u < p ? nb : r;
+}
+WDM_FI cx<float> ledger_scale(cx<float> a, double f, size_t i) {
+    return mk(ledger_round(double(a.x) * f, unsigned(2 * i)), ledger_round(double(a.y) * f, unsigned(2 * i + 1)));
+}
+WDM_FI cx<double> ledger_scale(cx<double> a, double f, size_t) { return a * f; }
+
+template <class T>
+WDM_KERNEL(256)
+void k_ledger_apply(cx<T>* __restrict__ psi, size_t n, cx<T>* __restrict__ hat, size_t nh,
+                    const double* __restrict__ ledger) {
+    const double f = ledger[1];
+    for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += size_t(gridDim.x) * 256)
+        psi[i] = ledger_scale(psi[i], f, i);
+    if (hat)
+        for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < nh; i += size_t(gridDim.x) * 256)
+            hat[i] = hat[i] * T(f * f);
 }
 
 WDM_KERNEL(64)
@@ -717,6 +770,19 @@ void ledger_update(wdm_plan* p) {
     WDM_LAUNCH((k_ledger_update), dim3(1), dim3(1024), p->stream, (const double*)p->partials.as<double>(),
                int(p->npartials / 2), p->ledger_buf.as<double>());
     hip_check(hipGetLastError(), "ledger update");
+}
+
+// Apply the pending scale to the state a call returns (psi, and Phi_hat when
+// it is valid), then clear it so that the next z pass does not apply it again.
+template <class T>
+void ledger_finish(wdm_plan* p) {
+    if (!p->ledger) return;
+    const size_t nh = (p->self_gravity && p->phi_valid) ? size_t(p->ny) * size_t(p->nz) * size_t(p->hpitch) : 0;
+    const unsigned nb = unsigned(std::min<size_t>((p->ncell + 255) / 256, 65536));
+    WDM_LAUNCH((k_ledger_apply<T>), dim3(nb), dim3(256), p->stream, p->psi.as<cx<T>>(), p->ncell,
+               nh ? p->rhohat.as<cx<T>>() : (cx<T>*)nullptr, nh, (const double*)p->ledger_buf.as<double>());
+    WDM_LAUNCH((k_ledger_set), dim3(1), dim3(64), p->stream, p->ledger_buf.as<double>(), 0.0, 0);
+    hip_check(hipGetLastError(), "ledger finish");
 }
 
 template <class T>
