@@ -38,6 +38,9 @@
  *                        (Prop.csv columns)                src/visualization/runtime.jl:210-238
  *   wdm_acceleration     grad_central(-dx, Phi_all)        src/simulation_main.jl:652
  *   wdm_density_*        time-averaged density buffer      src/simulation_main.jl:657-660
+ *   wdm_ic_velocity      solve_vector_equation + sqrt(rho) exp(i theta)
+ *                                                          src/physics/schrodinger.jl:8-30,
+ *                                                          src/ic/generation.jl:185-187,330-336
  */
 #ifndef WAVEDM_H
 #define WAVEDM_H
@@ -175,6 +178,16 @@ int wdm_step_timed(wdm_plan* plan, int nsteps, double dt, double dt_kick, double
    {x0, y0, z0, r_c, rho_c, phase, kx, ky, kz}.  Replaces evaluating the IC
    function on the host grid (src/physics/schrodinger.jl:37-46). */
 int wdm_ic_solitons(wdm_plan* plan, const double* params, int nsol);
+
+/* Initial condition from a density and a velocity field, all host arrays of
+   nx*ny*nz FP64 values, x-fastest: theta = the reference's
+   solve_vector_equation (src/physics/schrodinger.jl:8-30; three cumulative
+   trapezoid line integrals, each from the first cell of its axis, with the
+   plan's dx), psi = sqrt(rho) exp(i theta / (hbar/m)) (src/ic/generation.jl:
+   185-187).  remove_bulk = 1 first subtracts the rho-weighted mean velocity
+   (src/ic/generation.jl:330-336). */
+int wdm_ic_velocity(wdm_plan* plan, const double* rho, const double* vx, const double* vy, const double* vz,
+                    int remove_bulk);
 
 /* Adjoint SSFM (reference split).  No counterpart in the reference, which
    imports Zygote (src/WaveDM.jl:42) but defines no rule for the step; this is

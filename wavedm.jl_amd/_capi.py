@@ -89,6 +89,7 @@ SIGNATURES = {
     "wdm_density_average": (_I, [_P, _DP, C.POINTER(C.c_longlong)]),
     "wdm_density_reset": (_I, [_P]),
     "wdm_set_mass_ledger": (_I, [_P, _I]),
+    "wdm_ic_velocity": (_I, [_P, _DP, _DP, _DP, _DP, _I]),
     "wdm_cfl": (_I, [_P, _D, _DP]),
     "wdm_synchronize": (_I, [_P]),
     "wdm_step_timed": (_I, [_P, _I, _D, _D, _D, _DP, _I, _DP]),

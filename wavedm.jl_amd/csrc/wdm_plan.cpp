@@ -1390,3 +1390,4 @@ int wdm_synchronize(wdm_plan* p) {
 #include "wdm_adjoint.inl"
 #include "wdm_ensemble.inl"
 #include "wdm_diag.inl"
+#include "wdm_ic.inl"

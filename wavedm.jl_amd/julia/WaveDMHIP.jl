@@ -23,7 +23,8 @@ using ChainRulesCore
 export HIPPlan, ssfm, set_psi!, get_psi, set_border!, set_static_potential!, set_hernquist!,
        set_quadrupole!, set_kappa!, parallel_poisson, compute_gravitational_potential,
        apply_kick_step!, apply_drift_step!, step!, reduce_diagnostics, cfl_limits, step_properties,
-       accelerations, density_accumulate!, density_average, density_reset!, set_mass_ledger!
+       accelerations, density_accumulate!, density_average, density_reset!, set_mass_ledger!,
+       set_ic_velocity!
 
 const libwdm = get(ENV, "WAVEDM_HIP_LIB", joinpath(@__DIR__, "..", "lib", "libwavedm_hip.so"))
 
@@ -224,6 +225,15 @@ function accelerations(p::HIPPlan)
     check(ccall((:wdm_acceleration, libwdm), Cint, (Ptr{Cvoid}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}),
                 p.h, ax, ay, az), p.h)
     return ax, ay, az
+end
+
+"ψ = sqrt.(ρ) .* exp.(im*θ) with θ = solve_vector_equation(vx, vy, vz, Δ...) (src/physics/schrodinger.jl:8-30), on the device."
+function set_ic_velocity!(p::HIPPlan, ρ::Array{Float64,3}, vx::Array{Float64,3}, vy::Array{Float64,3},
+                          vz::Array{Float64,3}; remove_bulk::Bool=true)
+    check(ccall((:wdm_ic_velocity, libwdm), Cint,
+                (Ptr{Cvoid}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Cint),
+                p.h, ρ, vx, vy, vz, remove_bulk), p.h)
+    return p
 end
 
 "Opt-in mass projection for FP32 plans (no counterpart in the reference)."

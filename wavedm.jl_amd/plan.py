@@ -191,6 +191,18 @@ class Plan:
         prm = np.ascontiguousarray(np.asarray(params, dtype=np.float64).reshape(-1, 9))
         self._chk(self.lib.wdm_ic_solitons(self._h, _dptr(prm), prm.shape[0]))
 
+    def set_ic_velocity(self, rho, vx, vy, vz, remove_bulk=True):
+        """psi = sqrt(rho) exp(i theta / (hbar/m)), theta from the velocity field
+        by the reference's solve_vector_equation (src/physics/schrodinger.jl:8-30),
+        after removing the rho-weighted mean velocity (src/ic/generation.jl:330-336)."""
+        arrs = []
+        for a in (rho, vx, vy, vz):
+            a = np.asarray(a, dtype=np.float64)
+            if a.shape != self.shape:
+                raise ValueError(f"fields must have shape {self.shape}")
+            arrs.append(np.ascontiguousarray(a.ravel(order="F")))
+        self._chk(self.lib.wdm_ic_velocity(self._h, *[_dptr(a) for a in arrs], 1 if remove_bulk else 0))
+
     def reduce(self, n=A.WDM_RED_COUNT):
         out = np.zeros(A.WDM_RED_COUNT, dtype=np.float64)
         self._chk(self.lib.wdm_reduce(self._h, _dptr(out), int(n)))
