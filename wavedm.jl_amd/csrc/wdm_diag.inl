@@ -41,6 +41,31 @@ inline unsigned diag_blocks(int ny, int nz) {
     return std::max(1u, std::min((lines + kDiagWaves - 1) / kDiagWaves, 1024u));
 }
 
+// sqrt(r) and 1/r for finite r > 0 to a few 1e-15 relative: an FP32 hardware
+// seed (about 1 ulp) and one FP64 Newton step, a handful of FP64 operations
+// instead of the correctly rounded sequences.  Values outside the FP32 range
+// (and 0) take the IEEE operation.
+WDM_FI float diag_rsqf(float x) {
+#ifdef WDM_GPU
+    return __builtin_amdgcn_rsqf(x);
+#else
+    return 1.0f / __builtin_sqrtf(x);
+#endif
+}
+WDM_FI double diag_sqrt(double r) {
+    const float rf = float(r);
+    if (!(rf > 1.0e-37f) || !(rf < 1.0e37f)) return __builtin_sqrt(r);
+    const double y = double(diag_rsqf(rf));
+    const double sq = r * y;
+    return fmaT(0.5 * y, fmaT(-sq, sq, r), sq);
+}
+WDM_FI double diag_recip(double r) {
+    const float rf = float(r);
+    if (!(rf > 1.0e-37f) || !(rf < 1.0e37f)) return 1.0 / r;
+    const double y = double(rcpf(rf));
+    return fmaT(y, fmaT(-r, y, 1.0), y);
+}
+
 template <class T>
 WDM_FI double rho_of(cx<T> v) {
     const double a = double(v.x), b = double(v.y);
@@ -105,11 +130,11 @@ void k_diag_cells(const cx<T>* __restrict__ psi, const double* __restrict__ phi,
                     const double v = vm.flags ? eval_V<T>(vm, xs[x], ys[y], zs[z], i, c) : 0.0;
                     acc[4] += rho * ((phi ? phi[i] : 0.0) + v);
                 }
-                acc[5] += (rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2]) / rho;
+                acc[5] += (rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2]) * diag_recip(rho);
                 double q = 0.0;
                 WDM_UNROLL
                 for (int d = 0; d < 3; ++d) {
-                    const double s = (__builtin_sqrt(rho_of(n[2 * d + 1])) - __builtin_sqrt(rho_of(n[2 * d]))) * h[d];
+                    const double s = (diag_sqrt(rho_of(n[2 * d + 1])) - diag_sqrt(rho_of(n[2 * d]))) * h[d];
                     q += s * s;
                 }
                 acc[6] += q;
@@ -165,7 +190,7 @@ void k_diag_hist(const cx<T>* __restrict__ psi, int nx, int ny, int nz, const do
             const double rho = rho_of(psi[base + unsigned(x)]);
             if (rho > 0.0) {  // zero weights take no part (StatsBase drops them)
                 const double dx = xs[x] - c0;
-                const double r = __builtin_sqrt(dx * dx + r2yz);
+                const double r = diag_sqrt(dx * dx + r2yz);
                 const double fb = r * inv_width;
                 const int bin = fb < double(kDiagBins - 1) ? int(fb) : kDiagBins - 1;
                 atomicAdd(&hs[bin], (unsigned long long)__builtin_rint(rho * qscale));
