@@ -316,6 +316,24 @@ WDM_FI const E* stage_table(E* dst, const E* __restrict__ src) {
 struct LaneMap {
     int lane, wv, j, lw, w;
 };
+
+// Workgroup id with the XCD round robin undone: the hardware hands workgroup b
+// to XCD b mod 8, so consecutive ids land on different L2s.  Renumbering
+// b -> (b mod 8) q + b / 8 (q = gridDim / 8) gives each XCD a contiguous
+// range of logical ids, so neighbouring column groups share one L2.  The tail
+// beyond 8q keeps its id.
+#ifndef WDM_XCD_REMAP
+#define WDM_XCD_REMAP 1
+#endif
+WDM_FI unsigned xcd_block() {
+#if WDM_XCD_REMAP
+    const unsigned b = blockIdx.x, q = gridDim.x / 8u;
+    if (b >= q * 8u) return b;
+    return (b % 8u) * q + b / 8u;
+#else
+    return blockIdx.x;
+#endif
+}
 template <class F, int NW>
 WDM_FI LaneMap lane_map() {
     LaneMap m;
@@ -587,7 +605,8 @@ void k_col(cx<T>* __restrict__ data, int ncols, int nother, unsigned sline, unsi
     tw = stage_table<F::N>(tw_s, tw);
     const LaneMap m = lane_map<F, NW>();
     const int ntiles = (ncols + F::LPW - 1) / F::LPW;
-    const int o = m.w / ntiles, tile = m.w % ntiles;
+    const int wl = int(xcd_block()) * NW + m.wv;
+    const int o = wl / ntiles, tile = wl % ntiles;
     const int c = tile * F::LPW + m.lw;
     const bool ok = (o < nother) && (c < ncols);
     const int oc = ok ? o : 0, cc = ok ? c : 0;
