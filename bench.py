@@ -89,6 +89,7 @@ def run_gpu(W, cfg, precision, steps, warmup, ledger=False, diagnostics=True):
             p.step(warmup, dt)
             C.install(p, cfg)
         p.set_mass_ledger(ledger)
+        variants, _ = p.autotune(3, dt)  # per-axis pass tables measured on this state (state kept)
         red0 = p.reduce()
         p.synchronize()
         t0 = time.perf_counter()
@@ -97,7 +98,7 @@ def run_gpu(W, cfg, precision, steps, warmup, ledger=False, diagnostics=True):
         wall = time.perf_counter() - t0
         red1 = p.reduce()
         if not diagnostics:
-            return dict(dt=dt, wall=wall, red0=red0, red1=red1)
+            return dict(dt=dt, wall=wall, red0=red0, red1=red1, variants=variants)
         # the reference loop's per-step diagnostics on the final state (device)
         diag_ms = {}
         for label, nslots in (("full", W._capi.WDM_DIAG_COUNT), ("without_potential", W._capi.WDM_DIAG_PE_ABS)):
@@ -110,7 +111,7 @@ def run_gpu(W, cfg, precision, steps, warmup, ledger=False, diagnostics=True):
         diag = p.diagnostics()
         psi = p.get_psi(np.complex128)
     return dict(dt=dt, cfl=lims, wall=wall, event_ms=ev_ms, passes=passes, red0=red0, red1=red1, psi=psi,
-                diag=diag, diag_ms=diag_ms)
+                diag=diag, diag_ms=diag_ms, variants=variants)
 
 
 def cpu_baseline(W, cfg, dt, nsteps):
@@ -337,7 +338,8 @@ def main():
         "config": {"workload": f"{cfg['name']}: FP32 SSFM {cfg['shape'][0]}^3, L={2 * cfg['xmax'][0]:g} kpc, "
                                f"128 solitons, self-gravity + rotating tidal quadrupole, autoset dt, "
                                f"{args.steps} reference-split KDK steps",
-                   "grid": list(cfg["shape"]), "dt_gyr": r32["dt"], "cfl_limits": r32["cfl"]},
+                   "grid": list(cfg["shape"]), "dt_gyr": r32["dt"], "cfl_limits": r32["cfl"],
+                   "pass_variants_xyz": list(r32["variants"])},
         "hbm_gbs_step": step_bytes32 / (ms32 * 1e-3) / 1e9,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(cfg["shape"], dom),
