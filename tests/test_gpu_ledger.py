@@ -109,3 +109,33 @@ def test_projection_error_paths(wdm, nativelib):
         assert e.value.code == WDM_ERR_UNSUPPORTED
         p.set_mass_ledger(False)
         p.adjoint_forward(2, 1e-3)
+
+
+@pytest.mark.parametrize("graph", [True, False], ids=["graph", "eager"])
+def test_projection_in_an_ensemble(wdm, nativelib, graph):
+    # an ensemble step is one wdm_step of each member: with the projection on,
+    # members equal plans stepped one step per call, bit for bit
+    shapes = [(32, 32, 32), (16, 16, 16)]
+    dts = [2e-3, 1.5e-3]
+
+    def make(shape):
+        p = wdm.Plan(shape, X, "f32", four_pi_G=G4, lib=nativelib)
+        p.set_mass_ledger(True)
+        p.set_psi(_psi(shape))
+        return p
+
+    ens = [make(s) for s in shapes]
+    ref = [make(s) for s in shapes]
+    try:
+        m0 = [p.reduce(1)["mass"] for p in ens]
+        with wdm.Ensemble(ens) as e:
+            e.step(5, dts, graph=graph)
+        for p, dt in zip(ref, dts):
+            for k in range(5):
+                p.step(1, dt, t0=k * dt)
+        for a, b, m in zip(ens, ref, m0):
+            assert np.array_equal(a.get_psi(np.complex64), b.get_psi(np.complex64))
+            assert abs(a.reduce(1)["mass"] / m - 1) < 1e-7
+    finally:
+        for p in ens + ref:
+            p.close()
