@@ -139,3 +139,17 @@ def test_projection_in_an_ensemble(wdm, nativelib, graph):
     finally:
         for p in ens + ref:
             p.close()
+
+
+def test_projection_enabled_between_kick_and_drift_is_inert(wdm, nativelib):
+    # switched on after a kick taken without it: the drift scales by exactly 1
+    shape = (16, 16, 16)
+    out = []
+    for on in (True, False):
+        with wdm.Plan(shape, X, "f64", four_pi_G=G4, lib=nativelib) as p:
+            p.set_psi(_psi(shape))
+            p.kick(1e-3)
+            p.set_mass_ledger(on)
+            p.drift(2e-3)
+            out.append(p.get_psi())
+    assert np.array_equal(out[0], out[1])

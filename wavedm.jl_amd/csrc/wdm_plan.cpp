@@ -1200,7 +1200,13 @@ int wdm_set_mass_ledger(wdm_plan* p, int on) {
         return fail_code(p, WDM_ERR_UNSUPPORTED,
                          "wdm_set_mass_ledger: needs self-gravity (its density pass gives the mass) and no absorber");
     if (on && !p->ledger) {
-        const int rc = guarded(p, [&] { p->ledger_buf.get<double>(2); });
+        // [M_ref, f] = [0, 1] until the next kick or step sets the reference: a
+        // drift in between (after a kick taken without the projection) scales by 1
+        const int rc = guarded(p, [&] {
+            WDM_LAUNCH((k_ledger_set), dim3(1), dim3(64), p->stream, p->ledger_buf.get<double>(2), 0.0, 1);
+            hip_check(hipGetLastError(), "ledger init");
+            hip_check(hipStreamSynchronize(p->stream), "ledger init sync");
+        });
         if (rc != WDM_OK) return rc;
     }
     p->ledger = on != 0;
