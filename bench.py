@@ -79,7 +79,7 @@ def energy(red):
     return red["kinetic"] + red["w_self"] + red["v_ext"]
 
 
-def run_gpu(W, cfg, precision, steps, warmup, ledger=False, diagnostics=True):
+def run_gpu(W, cfg, precision, steps, warmup, ledger=False, diagnostics=True, round_ic=False):
     C = W.configs
     with W.Plan(cfg["shape"], cfg["xmax"], precision, hbar_over_m=cfg["hbar_over_m"],
                 four_pi_G=cfg["four_pi_G"]) as p:
@@ -89,16 +89,27 @@ def run_gpu(W, cfg, precision, steps, warmup, ledger=False, diagnostics=True):
             p.step(warmup, dt)
             C.install(p, cfg)
         p.set_mass_ledger(ledger)
+        if round_ic:  # the same initial state rounded to complex64 (a perturbation of FP32 rounding size)
+            p.set_psi(p.get_psi(np.complex64))
         variants, _ = p.autotune(3, dt)  # per-axis pass tables measured on this state (state kept)
+        # per-pass breakdown with events around every kernel (untimed: each event
+        # pair adds a few microseconds between kernels), then the timed region
+        # with events on the dominant pass only
+        npre = min(steps, 100)
+        breakdown, _ = p.step_timed(npre, dt)
+        dom = max(breakdown, key=breakdown.get)
         red0 = p.reduce()
         p.synchronize()
         t0 = time.perf_counter()
-        passes, ev_ms = p.step_timed(steps, dt)
+        timed, ev_ms = p.step_timed(steps, dt, t0=npre * dt, passes=[dom])
         p.synchronize()
         wall = time.perf_counter() - t0
+        passes = {k: v / npre * steps for k, v in breakdown.items()}  # per-launch averages, scaled to `steps`
+        passes[dom] = timed[dom]
         red1 = p.reduce()
         if not diagnostics:
-            return dict(dt=dt, wall=wall, red0=red0, red1=red1, variants=variants)
+            return dict(dt=dt, wall=wall, red0=red0, red1=red1, variants=variants,
+                        psi=p.get_psi(np.complex128) if round_ic else None)
         # the reference loop's per-step diagnostics on the final state (device)
         diag_ms = {}
         for label, nslots in (("full", W._capi.WDM_DIAG_COUNT), ("without_potential", W._capi.WDM_DIAG_PE_ABS)):
@@ -346,6 +357,8 @@ def main():
                      "algorithmic_bytes": bytes32[dom],
                      "step_frac": step_bytes32 / (ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "passes": passes,
+        "passes_note": "timed region: HIP events on the dominant pass only; the other passes from a "
+                       "preceding 100-step run with events on every kernel",
     }
     m0, m1 = r32["red0"]["mass"], r32["red1"]["mass"]
     out["fp32"] = {"mass_drift": abs(m1 - m0) / m0, "energy0": energy(r32["red0"]),
@@ -371,6 +384,12 @@ def main():
                        "mass_drift_fp64": abs(r64["red1"]["mass"] - r64["red0"]["mass"]) / r64["red0"]["mass"],
                        "energy_rel_diff_fp32_vs_fp64": abs(e32 - e64) / abs(e64),
                        "hbm_gbs_step": 2 * step_bytes32 / (ms64 * 1e-3) / 1e9}
+        # sensitivity of the workload: FP64 from the complex64-rounded initial state
+        print("[bench] FP64 leg from the FP32-rounded initial state", file=sys.stderr, flush=True)
+        rr = run_gpu(W, cfg, "f64", args.steps, min(args.warmup, 5), diagnostics=False, round_ic=True)
+        out["fp64"]["psi_rel_l2_fp64_from_rounded_ic"] = float(
+            np.linalg.norm(rr["psi"] - r64["psi"]) / np.linalg.norm(r64["psi"]))
+        del rr
         del r64
     del r32
     if not args.skip_adjoint:

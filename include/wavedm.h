@@ -170,6 +170,11 @@ int wdm_synchronize(wdm_plan* plan);
 #define WDM_PASS_COUNT 8
 int wdm_step_timed(wdm_plan* plan, int nsteps, double dt, double dt_kick, double t0, double* pass_ms,
                    int npass, double* total_ms);
+/* The same with events only on the passes whose bit (1u << WDM_PASS_*) is set
+   in pass_mask: each event pair costs a few microseconds between kernels, so a
+   timed region that needs one kernel's duration sets only that bit. */
+int wdm_step_timed_mask(wdm_plan* plan, int nsteps, double dt, double dt_kick, double t0, unsigned pass_mask,
+                        double* pass_ms, int npass, double* total_ms);
 
 /* Initial condition on the device (FP64 evaluation, stored in the plan
    precision): psi = sum_i sqrt(rho_c,i / (1 + 0.091 (r_i/r_c,i)^2)^8)

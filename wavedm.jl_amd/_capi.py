@@ -90,6 +90,7 @@ SIGNATURES = {
     "wdm_density_reset": (_I, [_P]),
     "wdm_set_mass_ledger": (_I, [_P, _I]),
     "wdm_ic_velocity": (_I, [_P, _DP, _DP, _DP, _DP, _I]),
+    "wdm_step_timed_mask": (_I, [_P, _I, C.c_double, C.c_double, C.c_double, C.c_uint, _DP, _I, _DP]),
     "wdm_cfl": (_I, [_P, _D, _DP]),
     "wdm_synchronize": (_I, [_P]),
     "wdm_step_timed": (_I, [_P, _I, _D, _D, _D, _DP, _I, _DP]),

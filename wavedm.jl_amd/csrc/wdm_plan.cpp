@@ -95,6 +95,7 @@ struct PassTimer {
     std::vector<hipEvent_t> ev;  // 2 per launch
     std::vector<int> id;
     size_t used = 0;             // launches recorded
+    unsigned mask = ~0u;         // passes that get events (bit WDM_PASS_*)
     void reserve(size_t launches) {
         while (ev.size() < 2 * launches) {
             hipEvent_t e;
@@ -157,15 +158,16 @@ namespace {
 
 // Arm the events for the next launch (call before it) / record which pass it
 // was (call after it).
-void arm(wdm_plan* p) {
+void arm(wdm_plan* p, int pass) {
 #ifdef WDM_GPU
     if (!p->timer) return;
     PassTimer& t = *p->timer;
-    if (t.used >= t.id.size()) return;
+    if (!((t.mask >> pass) & 1u) || t.used >= t.id.size()) return;
     launch_events().start = t.ev[2 * t.used];
     launch_events().stop = t.ev[2 * t.used + 1];
 #else
     (void)p;
+    (void)pass;
 #endif
 }
 void mark(wdm_plan* p, int pass) {
@@ -174,7 +176,7 @@ void mark(wdm_plan* p, int pass) {
 #endif
     if (!p->timer) return;
     PassTimer& t = *p->timer;
-    if (t.used >= t.id.size()) return;
+    if (!((t.mask >> pass) & 1u) || t.used >= t.id.size()) return;
     t.id[t.used++] = pass;
 }
 
@@ -269,7 +271,7 @@ void poisson_chain(wdm_plan* p, double scale) {
     const int nh = p->nx / 2 + 1, hp = p->hpitch;
     KTabs none = {nullptr, nullptr, nullptr, 0.0, nullptr};
     if (p->ny > 1) {
-        arm(p);
+        arm(p, WDM_PASS_PYFWD);
         oy->col(COL_FWD, KF_PNONE, rh, nh, p->nz, unsigned(hp), unsigned(hp * p->ny), p->ax[1].tw.as<TwE>(), none,
                 p->stream);
         mark(p, WDM_PASS_PYFWD);
@@ -277,12 +279,12 @@ void poisson_chain(wdm_plan* p, double scale) {
     // Phi_k = -scale * rho_k / k^2, with the 1/N of the unnormalised inverse folded in
     KTabs kt = {p->ax[0].k2.p, p->ax[1].k2.p, p->ax[2].k2.p,
                 -scale / double(p->ncell), nullptr};
-    arm(p);
+    arm(p, WDM_PASS_PZ);
     oz->col(COL_FMI, KF_POISSON, rh, nh, p->ny, unsigned(hp * p->ny), unsigned(hp), p->ax[2].tw.as<TwE>(), kt,
             p->stream);
     mark(p, WDM_PASS_PZ);
     if (p->ny > 1) {
-        arm(p);
+        arm(p, WDM_PASS_PYINV);
         oy->col(COL_INV, KF_PNONE, rh, nh, p->nz, unsigned(hp), unsigned(hp * p->ny), p->ax[1].tw.as<TwE>(), none,
                 p->stream);
         mark(p, WDM_PASS_PYINV);
@@ -406,7 +408,7 @@ void x_kick(wdm_plan* p, bool fwd, double dt_kick, double t) {
     typedef typename TwEntry<T>::type TwE;
     const AxisOps<T>* ox = ops_of<T>(p, 0);
     const KickArgs ka = make_kick<T>(p, dt_kick, t);
-    arm(p);
+    arm(p, WDM_PASS_XKICK);
     ox->x_kick(fwd, p->psi.as<cx<T>>(), p->rhohat.as<cx<T>>(), p->ny * p->nz, p->hpitch, p->ax[0].tw.as<TwE>(), ka,
                p->stream);
     hip_check(hipGetLastError(), "kick pass");
@@ -422,7 +424,7 @@ void y_pass(wdm_plan* p, int mode) {
     if (p->ny == 1) return;
     const AxisOps<T>* oy = ops_of<T>(p, 1);
     KTabs none = {nullptr, nullptr, nullptr, 0.0, nullptr};
-    arm(p);
+    arm(p, mode == COL_FWD ? WDM_PASS_YFWD : WDM_PASS_YINV);
     oy->col(mode, KF_NONE, p->psi.as<cx<T>>(), p->nx, p->nz, unsigned(p->nx), unsigned(p->nx * p->ny),
             p->ax[1].tw.as<TwE>(), none, p->stream);
     hip_check(hipGetLastError(), "y pass");
@@ -442,7 +444,7 @@ void z_pass(wdm_plan* p, int mode, double dt) {
     } else if (p->nz == 1) {
         return;
     }
-    arm(p);
+    arm(p, WDM_PASS_ZDRIFT);
     oz->col(mode, KF_NONE, p->psi.as<cx<T>>(), p->nx, p->ny, unsigned(p->nx * p->ny), unsigned(p->nx),
             p->ax[2].tw.as<TwE>(), kt, p->stream);
     hip_check(hipGetLastError(), "z pass");
@@ -454,7 +456,7 @@ template <class T>
 void x_inv_close(wdm_plan* p) {
     typedef typename TwEntry<T>::type TwE;
     const AxisOps<T>* ox = ops_of<T>(p, 0);
-    arm(p);
+    arm(p, WDM_PASS_XINV);
     ox->x_inv_dens(p->psi.as<cx<T>>(), p->self_gravity ? p->rhohat.as<cx<T>>() : nullptr, p->ny * p->nz,
                    p->hpitch, p->ax[0].tw.as<TwE>(), p->border_on ? p->border.as<double>() : nullptr,
                    p->partials.as<double>(), p->stream);
@@ -1066,12 +1068,16 @@ int guarded(wdm_plan* p, F&& f) {
 // nsteps reference-split steps with HIP events around every pass launch:
 // acc[pass] += summed kernel ms, *total = first-to-last event time.  Set-up
 // work (Phi of the initial psi, drift tables) runs before the timed window.
-void timed_steps(wdm_plan* p, int nsteps, double dt, double dt_kick, double t0, double* acc, double* total) {
+void timed_steps(wdm_plan* p, int nsteps, double dt, double dt_kick, double t0, double* acc, double* total,
+                 unsigned mask = ~0u) {
     PassTimer timer;
+    timer.mask = mask;
     DISPATCH(p, ensure_phi, p);
     ensure_drift(p, dt);
     hip_check(hipStreamSynchronize(p->stream), "timed sync");
-    timer.reserve(size_t(WDM_PASS_COUNT) * nsteps);
+    int npass = 0;
+    for (int i = 0; i < WDM_PASS_COUNT; ++i) npass += (mask >> i) & 1u;
+    timer.reserve(size_t(npass) * nsteps);
     p->timer = &timer;
     try {
         DISPATCH(p, do_step, p, nsteps, dt, dt_kick, t0, WDM_SPLIT_REFERENCE);
@@ -1381,6 +1387,20 @@ int wdm_step_timed(wdm_plan* p, int nsteps, double dt, double dt_kick, double t0
     return guarded(p, [&] {
         double acc[WDM_PASS_COUNT] = {0}, tot = 0.0;
         timed_steps(p, nsteps, dt, dt_kick, t0, acc, &tot);
+        for (int i = 0; i < npass && i < WDM_PASS_COUNT; ++i) pass_ms[i] = acc[i];
+        if (total_ms) *total_ms = tot;
+    });
+}
+
+int wdm_step_timed_mask(wdm_plan* p, int nsteps, double dt, double dt_kick, double t0, unsigned pass_mask,
+                        double* pass_ms, int npass, double* total_ms) {
+    if (!p) return fail_code(nullptr, WDM_ERR_ARG, "plan is NULL");
+    if (nsteps < 0 || !std::isfinite(dt) || !std::isfinite(dt_kick) || !std::isfinite(t0))
+        return fail_code(p, WDM_ERR_ARG, "wdm_step_timed_mask: bad argument");
+    if (p->kspace) return fail_code(p, WDM_ERR_STATE, "wdm_step_timed_mask: psi is in k-space");
+    return guarded(p, [&] {
+        double acc[WDM_PASS_COUNT] = {0}, tot = 0.0;
+        timed_steps(p, nsteps, dt, dt_kick, t0, acc, &tot, pass_mask);
         for (int i = 0; i < npass && i < WDM_PASS_COUNT; ++i) pass_ms[i] = acc[i];
         if (total_ms) *total_ms = tot;
     });

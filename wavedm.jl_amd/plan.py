@@ -175,16 +175,24 @@ class Plan:
         mode = {"reference": A.WDM_SPLIT_REFERENCE, "strang": A.WDM_SPLIT_STRANG}[split]
         self._chk(self.lib.wdm_step(self._h, int(nsteps), float(dt), float(dt_kick), float(t0), mode))
 
-    def step_timed(self, nsteps, dt, dt_kick=None, t0=0.0):
-        """Reference-split steps with HIP events around every pass; returns
-        (per-pass summed ms dict, total event ms)."""
+    def step_timed(self, nsteps, dt, dt_kick=None, t0=0.0, passes=None):
+        """Reference-split steps with HIP events around every pass (or only the
+        passes named in ``passes``); returns (per-pass summed ms dict, total
+        event ms)."""
         if dt_kick is None:
             dt_kick = 0.5 * dt
         ms = np.zeros(len(A.PASS_NAMES), dtype=np.float64)
         tot = np.zeros(1, dtype=np.float64)
-        self._chk(self.lib.wdm_step_timed(self._h, int(nsteps), float(dt), float(dt_kick), float(t0),
-                                          _dptr(ms), len(ms), _dptr(tot)))
-        return dict(zip(A.PASS_NAMES, ms.tolist())), float(tot[0])
+        if passes is None:
+            self._chk(self.lib.wdm_step_timed(self._h, int(nsteps), float(dt), float(dt_kick), float(t0),
+                                              _dptr(ms), len(ms), _dptr(tot)))
+            return dict(zip(A.PASS_NAMES, ms.tolist())), float(tot[0])
+        mask = 0
+        for name in passes:
+            mask |= 1 << A.PASS_NAMES.index(name)
+        self._chk(self.lib.wdm_step_timed_mask(self._h, int(nsteps), float(dt), float(dt_kick), float(t0),
+                                               mask, _dptr(ms), len(ms), _dptr(tot)))
+        return {n: ms[A.PASS_NAMES.index(n)] for n in passes}, float(tot[0])
 
     def set_ic_solitons(self, params):
         """Device-side soliton superposition (rows x0,y0,z0,r_c,rho_c,phase,kx,ky,kz)."""
