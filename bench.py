@@ -311,8 +311,8 @@ def main():
                     help="only run a plain device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the multi-GPU slab decomposition is a design note, not part of this build", file=sys.stderr)
+        print(f"bench.py: this library runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU slab decomposition is a design note, not implemented", file=sys.stderr)
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration(args.copy_calibration, max(args.steps, 1))

@@ -17,7 +17,7 @@
  *
  * Reference interfaces replaced (paths relative to the WaveDM.jl tree):
  *   wdm_plan_create      setup_grid / setup_coordinates   src/physics/poisson.jl:8-28
- *                        + the driver's buffer set-up      src/simulation_main.jl:212-214,414-417
+ *                        + the loop's buffer set-up        src/simulation_main.jl:212-214,414-417
  *   wdm_set_psi          DeviceArray(psi) upload           src/simulation_main.jl:417
  *   wdm_get_psi          collect(device_psi)               src/simulation_main.jl:551
  *   wdm_set_border       setup_absorption_boundary         src/solver/fft_operators.jl:22-27

@@ -1,8 +1,8 @@
 """FP64 CPU adjoint of the reference-split SSFM step -- test infrastructure.
 
 Only tests/ (and bench.py's CPU leg) may import this module; it is the
-checker of the device adjoint (wdm_adjoint_forward / wdm_adjoint_backward),
-never part of the product path.
+reference the device adjoint (wdm_adjoint_forward / wdm_adjoint_backward) is
+compared against, never part of the product path.
 
 The forward map restates ssfm_oracle.run_ssfm with split=SPLIT_REFERENCE
 (src/simulation_main.jl:536-547, src/solver/KDK.jl:37-101,

@@ -2,7 +2,7 @@
 
 TEST INFRASTRUCTURE ONLY.  Nothing in the shipped library imports this module;
 only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline leg
-use it, always as the checker (or as the timed CPU restatement), never as the
+use it, always to check results (or as the timed CPU restatement), never as the
 thing measured on the GPU.
 
 This is a restatement, op for op, of the reference's CPU path (FFTW replaced by
@@ -184,7 +184,7 @@ def setup_fft_operators(Xmax, Ymax, Zmax, Nx, Ny, Nz, dt, hbar_over_m=1.0):
 
 
 def setup_absorption_boundary(Xmax, Ymax, Zmax, x, y, z, absorb_coeff, dt):
-    """src/solver/fft_operators.jl:22-27 called with 1-D x,y,z as the driver does
+    """src/solver/fft_operators.jl:22-27 called with 1-D x,y,z as the simulation loop does
     (simulation_main.jl:415): the result is a length-Nx vector whose element i
     mixes x[i], y[i], z[i] and is broadcast along dim 1 only."""
     wx, wy, wz = Xmax / 50.0, Ymax / 50.0, Zmax / 50.0

@@ -240,7 +240,7 @@ def apply_drift_step(spec: Plan, linear_phase: DriftOperator, border=None, gpu=T
 
 def run_loop(plan: Plan, Nt, dt, *, KDK_flag=True, t0=0.0, snapshot_every=0, on_snapshot=None,
              split="reference"):
-    """The driver loop (Nt-1 iterations of K(dt_kick) D(dt), Q1/Q2) on the fused
+    """The simulation loop (Nt-1 iterations of K(dt_kick) D(dt), Q1/Q2) on the fused
     device path, copying psi to the host only at snapshot cadence."""
     dt_kick = 0.5 * dt if KDK_flag else dt
     nsteps = Nt - 1
