@@ -102,7 +102,10 @@ def _check(dev, ref, psi, X, shape, pe_tol, sharp=True):
 
 
 CASES = [((32, 24, 16), (3.0, 2.5, 2.0), True, True), ((64, 32, 48), (4.0, 3.0, 3.5), False, True),
-         ((16, 12, 8), (2.0, 1.5, 1.0), True, False)]
+         ((16, 12, 8), (2.0, 1.5, 1.0), True, False),
+         # rows longer than one wave (x neighbours across 64-column chunks, a partial chunk)
+         # and a y extent that is not a multiple of the 32-row walk
+         ((96, 48, 12), (4.0, 3.0, 2.0), True, True)]
 IDS = ["x".join(map(str, c[0])) for c in CASES]
 
 
