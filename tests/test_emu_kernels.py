@@ -18,7 +18,7 @@ SUBSET = ("roundtrip or (poisson and (8x8x8 or 32x16x8 or 160x16x8 or 12x8x384))
           "ensemble_equals or recaptures or time_dependent or (every_variant and (64x8x8 or 32x16x8)) or "
           "device_matches_golden or (diagnostics_match_oracle and (32x24x16 or 16x12x8 or 96x48x12)) or "
           "diagnostics_after or diagnostics_error or restart_continues or density_average or "
-          "projection_error_paths or projection_reference_follows or projection_enabled or ic_velocity")
+          "projection_error_paths or projection_reference_follows or projection_enabled or ic_velocity or baryon_mesh")
 
 
 def test_gpu_suite_on_emulated_kernels(wdm):

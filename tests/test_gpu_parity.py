@@ -112,6 +112,43 @@ def test_kick_and_drift_api_match_oracle(wdm, nativelib):
     assert rel_max(out, out_ref) < 1e-12
 
 
+def test_baryon_mesh_potential_and_kdk_match_oracle(wdm, nativelib):
+    """baryon_mode :mesh: Phi_b = 4 pi G lap^-1 rho_b (src/simulation_main.jl:244) solved on
+    the device, added to Phi_WaveDM in every kick (src/solver/gravity.jl:24-28) through the
+    mirror's kick/drift calls and through the fused loop."""
+    shape, dt, nsteps = (32, 16, 8), 2e-3, 3
+    psi = blob(shape, X3, seed=5)
+    x, y, z, delta, _ = O.setup_grid(*X3, *shape)
+    r2 = (x[:, None, None] - 0.4) ** 2 + y[None, :, None] ** 2 + (z[None, None, :] + 0.2) ** 2
+    rho_b = 40.0 * (1.0 + r2 / 0.6 ** 2) ** (-1.5 * 0.8)  # a cored beta-model-shaped density
+    phi_b_ref = (4 * np.pi * O.parallel_poisson(delta, tuple(n - 1 for n in shape), rho_b)).real
+    K = wdm.kdk
+    lp = K.setup_fft_operators(*X3, *shape, dt)
+    with wdm.Plan(shape, X3, "f64", lib=nativelib) as p:
+        p.set_psi(psi)
+        phi_b = K.baryon_mesh_potential(p, rho_b)
+        cg = K.GravityConfig(baryon_mode=":mesh", Phi_b=phi_b)
+        for s in range(nsteps):
+            phi_all, phi_wdm, spec = K.apply_kick_step(p, config_gravity=cg, dt=dt / 2, t=s * dt)
+            if s == 0:
+                phi_all0, phi_wdm0 = phi_all, phi_wdm
+            K.apply_drift_step(spec, lp, None)
+        out = p.get_psi()
+    with wdm.Plan(shape, X3, "f64", lib=nativelib) as q:
+        q.set_psi(psi)
+        K.run_loop(q, nsteps + 1, dt, config_gravity=cg)
+        fast = q.get_psi()
+    assert rel_max(phi_b, phi_b_ref) < 1e-12, rel_max(phi_b, phi_b_ref)
+    phi_ref = O.compute_gravitational_potential(psi, delta).real
+    assert rel_max(phi_wdm0, phi_ref) < 1e-12
+    assert rel_max(phi_all0, phi_ref + phi_b_ref) < 1e-12
+    ref = O.run_ssfm(psi, X3, nsteps, dt, V=phi_b_ref)
+    assert rel_l2(out, ref) < 1e-12, rel_l2(out, ref)
+    assert rel_l2(fast, ref) < 1e-12, rel_l2(fast, ref)
+    # the baryon term matters at this size: dropping it moves psi far beyond the bar
+    assert rel_l2(O.run_ssfm(psi, X3, nsteps, dt), ref) > 1e-6
+
+
 def test_c0_config_100_steps_fp64(wdm, nativelib):
     """64^3, L=20 kpc, m22=1 soliton with a 0.3 rad random phase field,
     dt=1e-3 Gyr (Float32-rounded as the reference does), 100 reference-split

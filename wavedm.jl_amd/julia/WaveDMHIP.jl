@@ -24,7 +24,7 @@ export HIPPlan, ssfm, set_psi!, get_psi, set_border!, set_static_potential!, set
        set_quadrupole!, set_kappa!, parallel_poisson, compute_gravitational_potential,
        apply_kick_step!, apply_drift_step!, step!, reduce_diagnostics, cfl_limits, step_properties,
        accelerations, density_accumulate!, density_average, density_reset!, set_mass_ledger!,
-       set_ic_velocity!
+       set_ic_velocity!, baryon_mesh_potential
 
 const libwdm = get(ENV, "WAVEDM_HIP_LIB", joinpath(@__DIR__, "..", "lib", "libwavedm_hip.so"))
 
@@ -153,11 +153,27 @@ function parallel_poisson(p::HIPPlan, ρ::Array{Float64,3}; scale::Real=1.0)
     return Φ
 end
 
-"Φ = 4πG ∇⁻²(|ψ|² - ⟨|ψ|²⟩) of the plan's ψ (src/solver/gravity.jl:13-23), as (Φ_all, Φ_WaveDM)."
-function compute_gravitational_potential(p::HIPPlan)
+"""
+    baryon_mesh_potential(p::HIPPlan, ρ_baryon; four_pi_G=0.0) -> Φ_b
+
+Φ_b = 4πG ∇⁻²(ρ_baryon - ⟨ρ_baryon⟩) for `baryon_mode = :mesh`
+(src/simulation_main.jl:244) on the plan's lattice and precision, real part.
+`four_pi_G = 0` takes the plan's coefficient.  Install it with
+`set_static_potential!(p, Φ_b)` so every kick adds it (src/solver/gravity.jl:24-28).
+"""
+baryon_mesh_potential(p::HIPPlan, ρ_baryon::Array{Float64,3}; four_pi_G::Real=0.0) =
+    parallel_poisson(p, ρ_baryon; scale=four_pi_G)
+
+"""
+    compute_gravitational_potential(p::HIPPlan; Φ_b=nothing) -> (Φ_all, Φ_WaveDM)
+
+Φ_WaveDM = 4πG ∇⁻²(|ψ|² - ⟨|ψ|²⟩) of the plan's ψ (src/solver/gravity.jl:13-23);
+Φ_all adds the mesh baryon potential `Φ_b` when one is given (gravity.jl:24-28).
+"""
+function compute_gravitational_potential(p::HIPPlan; Φ_b::Union{Nothing,Array{Float64,3}}=nothing)
     Φ = Array{Float64,3}(undef, p.dims)
     check(ccall((:wdm_get_phi, libwdm), Cint, (Ptr{Cvoid}, Ptr{Cdouble}), p.h, Φ), p.h)
-    return Φ, Φ
+    return (Φ_b === nothing ? Φ : Φ .+ Φ_b), Φ
 end
 
 """

@@ -10,6 +10,7 @@ time-step helpers are plain host logic, as in the reference.
   setup_fft_operators              src/solver/fft_operators.jl:8-15
   setup_absorption_boundary        src/solver/fft_operators.jl:22-27
   parallel_poisson                 src/parallel_poisson.jl:36-57
+  baryon_mesh_potential            src/simulation_main.jl:244 (baryon_mode = :mesh)
   compute_gravitational_potential  src/solver/gravity.jl:7-35
   apply_kick_step                  src/solver/KDK.jl:37-77
   apply_drift_step                 src/solver/KDK.jl:88-101
@@ -112,10 +113,32 @@ class DeviceConfig:
 
 @dataclass
 class GravityConfig:
-    """Periodic self-gravity with baryon_mode = :ignored (src/core/configs.jl:161-170)."""
+    """Periodic self-gravity (src/core/configs.jl:161-170) with baryon_mode
+    :ignored, or :mesh with a fixed baryon potential ``Phi_b`` added to
+    Phi_WaveDM in every kick (src/solver/gravity.jl:24-28).  ``Phi_b`` is the
+    real field from ``baryon_mesh_potential`` (src/simulation_main.jl:244); a
+    complex one (the reference's Poisson output) contributes its real part."""
     boundary: str = "periodic"
     baryon_mode: str = "ignored"
     four_pi_G: float = 4.0 * math.pi
+    Phi_b: np.ndarray | None = None
+
+    def __post_init__(self):
+        mode = str(self.baryon_mode).lstrip(":")
+        if mode not in ("ignored", "mesh"):
+            raise ValueError(f"Unsupported baryon_mode {self.baryon_mode}: the HIP path takes :ignored or :mesh")
+        self.baryon_mode = mode
+        if mode == "mesh":
+            if self.Phi_b is None:
+                raise ValueError("baryon_mode :mesh needs the baryon potential Phi_b")
+            phi_b = np.asarray(self.Phi_b)
+            if phi_b.ndim != 3:
+                raise ValueError("Phi_b must be a 3-D field on the simulation grid")
+            self.Phi_b = np.asarray(phi_b.real, dtype=np.float64)
+
+    def baryon_potential(self):
+        """Phi_b when baryons are on the mesh, else None."""
+        return self.Phi_b if self.baryon_mode == "mesh" else None
 
 
 @dataclass
@@ -159,28 +182,37 @@ class Kappa:
     kappa: float
 
 
-def _install_potential(plan: Plan, V, t):
+def _install_potential(plan: Plan, V, t, phi_b=None):
     """Device models are installed once; a host callable is evaluated each
-    kick on the host psi (the reference's V.(xxx,yyy,zzz,psi), KDK.jl:64)."""
-    items = V if isinstance(V, (list, tuple)) else [V]
-    plan.clear_potential_models()
-    plan.set_static_potential(None)
-    for v in items:
-        if v is None:
-            continue
-        if isinstance(v, Hernquist):
-            plan.set_hernquist(v.M, v.a, v.center)
-        elif isinstance(v, RotatingQuadrupole):
-            plan.set_quadrupole(v.T0, v.axis, v.period, v.center)
-        elif isinstance(v, Kappa):
-            plan.set_kappa(v.kappa)
-        elif callable(v):
-            x, y, z, _, _ = setup_grid(*plan.xmax, *plan.shape, consistent=plan.consistent_grid)
-            xxx, yyy, zzz, _ = setup_coordinates(x, y, z)
-            psi = plan.get_psi()
-            plan.set_static_potential(np.asarray(v(xxx, yyy, zzz, psi), dtype=np.float64).real)
-        else:
-            plan.set_static_potential(np.asarray(v, dtype=np.float64))
+    kick on the host psi (the reference's V.(xxx,yyy,zzz,psi), KDK.jl:64).
+    The static field the kick adds is the sum of the array and callable
+    items and the baryon potential ``phi_b``.  V = None leaves the plan's
+    device models as they are."""
+    static = phi_b
+    if V is not None:
+        items = V if isinstance(V, (list, tuple)) else [V]
+        plan.clear_potential_models()
+        for v in items:
+            if v is None:
+                continue
+            if isinstance(v, Hernquist):
+                plan.set_hernquist(v.M, v.a, v.center)
+                continue
+            if isinstance(v, RotatingQuadrupole):
+                plan.set_quadrupole(v.T0, v.axis, v.period, v.center)
+                continue
+            if isinstance(v, Kappa):
+                plan.set_kappa(v.kappa)
+                continue
+            if callable(v):
+                x, y, z, _, _ = setup_grid(*plan.xmax, *plan.shape, consistent=plan.consistent_grid)
+                xxx, yyy, zzz, _ = setup_coordinates(x, y, z)
+                field_v = np.asarray(v(xxx, yyy, zzz, plan.get_psi())).real
+            else:
+                field_v = np.asarray(v).real
+            field_v = np.asarray(field_v, dtype=np.float64)
+            static = field_v if static is None else static + field_v
+    plan.set_static_potential(static)
 
 
 # ---------------------------------------------------------------- solver API
@@ -206,10 +238,27 @@ def parallel_poisson(delta, N, rho, boundary="periodic", backend=None, gpu_force
     return phi.astype(np.complex128)
 
 
+def baryon_mesh_potential(plan: Plan, rho_baryon, four_pi_G=None):
+    """Phi_b = 4 pi G * parallel_poisson(Delta, [Nx-1, Ny-1, Nz-1], rho_baryon)
+    for baryon_mode = :mesh (src/simulation_main.jl:244, src/ic/generation.jl:54),
+    solved on ``plan``'s device lattice (period 2 Xmax per axis, as the
+    reference's N Delta) in the plan's precision; real part, host FP64.
+    ``four_pi_G`` None takes the plan's coefficient (the reference's literal 4 pi
+    in code units).  The plan's psi is left as it was."""
+    rho = np.asarray(rho_baryon, dtype=np.float64)
+    if rho.shape != plan.shape:
+        raise ValueError(f"rho_baryon must have shape {plan.shape}, got {rho.shape}")
+    return plan.poisson(rho, scale=0.0 if four_pi_G is None else float(four_pi_G))
+
+
 def compute_gravitational_potential(device_psi: Plan, config_gravity=None, grid=None, config_device=None):
     """rho = |psi|^2; rho -= mean(rho); Phi = 4 pi G lap^-1 rho (gravity.jl:13-23).
-    Returns host (Phi_all, Phi_WaveDM); with baryons ignored they are the same array."""
+    Returns host (Phi_all, Phi_WaveDM): the same array with baryons ignored,
+    Phi_all = Phi_WaveDM + Phi_b with baryons on the mesh (gravity.jl:24-28)."""
     phi = device_psi.get_phi()
+    phi_b = config_gravity.baryon_potential() if config_gravity is not None else None
+    if phi_b is not None:
+        return phi + phi_b, phi
     return phi, phi
 
 
@@ -222,8 +271,9 @@ def apply_kick_step(device_psi: Plan, psi=None, V=None, rho=None, rho_max=None, 
     tk = 0.0
     if t is not None:
         tk = float(t[i - 1]) if hasattr(t, "__len__") else float(t)
-    if V is not None:
-        _install_potential(device_psi, V, tk)
+    phi_b = config_gravity.baryon_potential() if config_gravity is not None else None
+    if V is not None or phi_b is not None:
+        _install_potential(device_psi, V, tk, phi_b)
     phi_all = phi_wdm = None
     if want_potential:
         phi_all, phi_wdm = compute_gravitational_potential(device_psi, config_gravity, grid, config_device)
@@ -239,9 +289,14 @@ def apply_drift_step(spec: Plan, linear_phase: DriftOperator, border=None, gpu=T
 
 
 def run_loop(plan: Plan, Nt, dt, *, KDK_flag=True, t0=0.0, snapshot_every=0, on_snapshot=None,
-             split="reference"):
+             split="reference", config_gravity=None):
     """The simulation loop (Nt-1 iterations of K(dt_kick) D(dt), Q1/Q2) on the fused
-    device path, copying psi to the host only at snapshot cadence."""
+    device path, copying psi to the host only at snapshot cadence.  With baryons
+    on the mesh, Phi_b is uploaded once as the plan's static potential (it
+    replaces a static field set before); device potential models stay."""
+    phi_b = config_gravity.baryon_potential() if config_gravity is not None else None
+    if phi_b is not None:
+        plan.set_static_potential(phi_b)
     dt_kick = 0.5 * dt if KDK_flag else dt
     nsteps = Nt - 1
     done = 0
