@@ -1,6 +1,6 @@
-"""The 384^3 workload (128 solitons, rotating tidal quadrupole, self-gravity,
-autoset dt) at its full size, checked through properties that do not depend
-on the grid size; the CPU oracle is too slow at this size.
+"""The workloads at their full sizes, checked through properties that do not
+depend on the grid size; the CPU oracle is too slow at these sizes.  The 384^3
+workload (128 solitons, rotating tidal quadrupole, self-gravity, autoset dt):
 
 * Time reversal: one reference step is D(dt) K(dt_kick).  Its inverse is
   K(-dt_kick) D(-dt), because |K psi| = |psi| leaves Phi unchanged.  Running
@@ -10,7 +10,11 @@ on the grid size; the CPU oracle is too slow at this size.
   FP64 steps.
 * FP32 against FP64 from the same (FP32-rounded) psi0 after 5 steps: relative
   L2 <= 1e-5, the FP32 precision form's bar for psi, at a horizon where the
-  dynamics do not yet amplify rounding."""
+  dynamics do not yet amplify rounding.
+
+The adjoint workload (128^3, 200 steps): gradient against central
+differences.  The batched workload (48 boxes): every box equals its single
+run bit for bit."""
 import numpy as np
 import pytest
 
@@ -81,3 +85,85 @@ def test_c2_fp32_tracks_fp64_short_horizon(wdm, nativelib, c2):
     err = _rel_l2(out32, out64)
     print(f"  rel L2 {err:.3e}", flush=True)
     assert err < 1e-5, err
+
+
+def test_c4_gradient_matches_central_differences(wdm, nativelib):
+    """The adjoint workload at its size (128^3, 16 solitons, Hernquist host,
+    200 steps, checkpoints every 20, 40-bin radial profile loss against a run
+    with perturbed host parameters) on an FP64 plan: the device gradient
+    against central differences of the device forward, for a random relative
+    perturbation of psi0 and every parameter, <= 1e-4 relative.  Each
+    difference is Richardson-extrapolated from steps h = 1e-3 (relative) and
+    h/2: over 200 steps the plain difference at h = 1e-3 is off by ~5e-4,
+    falling as h^2, which the extrapolation removes."""
+    print("c4 128^3 f64: gradient vs central differences, 200 steps", flush=True)
+    cfg = wdm.configs.build("c4")
+    nsteps = cfg["steps"]
+    hq = cfg["potential"][1]
+    HQ = (hq["M"], hq["a"], *hq["center"])
+    loss = wdm.adjoint.ProfileLoss(cfg["shape"], cfg["xmax"], nbins=40)
+    with _plan(wdm, nativelib, cfg, "f64") as p:
+        dt, _ = wdm.configs.autoset_dt(p, nsteps)
+        psi0 = p.get_psi()
+
+        def fwd(psi, par=HQ, sigma=1.0):
+            p.clear_potential_models()
+            p.set_hernquist(par[0], par[1], par[2:])
+            p.set_psi(psi)
+            p.step(nsteps, sigma * dt)
+            return p.get_psi()
+
+        target = loss.profile(fwd(psi0, (HQ[0] * 1.05, HQ[1] * 0.95, HQ[2] + 0.5, 0.2, -0.3)))
+        p.clear_potential_models()
+        p.set_hernquist(HQ[0], HQ[1], HQ[2:])
+        L, g0, gp = wdm.adjoint.value_and_grad(p, psi0, nsteps, dt, loss, target, ckpt_every=20)
+
+        def slope(g, h):
+            d1 = (g(h) - g(-h)) / (2 * h)
+            d2 = (g(h / 2) - g(-h / 2)) / h
+            return (4 * d2 - d1) / 3, d1
+
+        f = lambda *a, **k: loss(fwd(*a, **k), target)[0]  # noqa: E731
+        # a relative perturbation of psi0: a direction of uniform amplitude would
+        # dominate the outer shells, where psi0 is ~0 and the profile loss is
+        # quadratic in it
+        rng = np.random.default_rng(5)
+        d = psi0 * (rng.normal(size=psi0.shape) + 1j * rng.normal(size=psi0.shape))
+        fd, d1 = slope(lambda s: f(psi0 + s * d), 1e-3 * np.linalg.norm(psi0) / np.linalg.norm(d))
+        ad = float(np.sum(np.real(np.conj(g0) * d)))
+        print(f"  L {L:.6e}  psi0: fd {fd:.8e} (h: {d1:.8e}) ad {ad:.8e}", flush=True)
+        assert abs(fd - ad) <= 1e-4 * abs(fd), ("psi0", fd, d1, ad)
+        par = np.array(HQ)
+        for j in range(5):
+            e = np.zeros(5)
+            e[j] = 1.0
+            fd, d1 = slope(lambda s: f(psi0, tuple(par + s * e)), 1e-3 * max(abs(par[j]), 1.0))
+            print(f"  {wdm.adjoint.PARAM_NAMES[j]}: fd {fd:.8e} (h: {d1:.8e}) ad {gp[j]:.8e}", flush=True)
+            assert abs(fd - gp[j]) <= 1e-4 * abs(fd), (wdm.adjoint.PARAM_NAMES[j], fd, d1, gp[j])
+        fd, d1 = slope(lambda s: f(psi0, sigma=1 + s), 1e-3)
+        print(f"  sigma: fd {fd:.8e} (h: {d1:.8e}) ad {gp[5]:.8e}", flush=True)
+        assert abs(fd - gp[5]) <= 1e-4 * abs(fd), ("sigma", fd, d1, gp[5])
+
+
+def test_c3_ensemble_members_equal_single_runs(wdm, nativelib):
+    """The batched workload at its size (48 FP32 boxes: 16 each of 64^3, 128^3
+    and 160^3, self-gravity, autoset dt per box): every box stepped in the
+    ensemble's graph equals the same box stepped alone, bit for bit."""
+    print("c3 48 boxes f32: graph ensemble vs single runs, 3 + 2 steps", flush=True)
+    cfg = wdm.configs.build("c3")
+    ens, ref, dts = [], [], []
+    try:
+        for m in cfg["members"]:
+            for lst in (ens, ref):
+                lst.append(_plan(wdm, nativelib, m, "f32"))
+            dts.append(wdm.configs.autoset_dt(ens[-1], cfg["steps"])[0])
+        with wdm.Ensemble(ens) as e:
+            e.step(3, dts, graph=True)
+            e.step(2, dts, graph=True)  # replays the captured step
+        for p, dt in zip(ref, dts):
+            p.step(5, dt)
+        same = [np.array_equal(a.get_psi(np.complex64), b.get_psi(np.complex64)) for a, b in zip(ens, ref)]
+    finally:
+        for p in ens + ref:
+            p.close()
+    assert all(same), [i for i, s in enumerate(same) if not s]
