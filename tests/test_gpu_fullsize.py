@@ -1,6 +1,7 @@
 """The workloads at their full sizes, checked through properties that do not
-depend on the grid size; the CPU oracle is too slow at these sizes.  The 384^3
-workload (128 solitons, rotating tidal quadrupole, self-gravity, autoset dt):
+depend on the grid size; the CPU oracle is too slow at these sizes.  The 256^3
+workload (64 solitons, Hernquist host, self-gravity) and the 384^3 workload
+(128 solitons, rotating tidal quadrupole, self-gravity), both with autoset dt:
 
 * Time reversal: one reference step is D(dt) K(dt_kick).  Its inverse is
   K(-dt_kick) D(-dt), because |K psi| = |psi| leaves Phi unchanged.  Running
@@ -34,16 +35,22 @@ def _plan(wdm, nativelib, cfg, prec):
     return p
 
 
-@pytest.fixture(scope="module")
-def c2(wdm):
-    return wdm.configs.build("c2")
+_CFGS = {}
 
 
+def _cfg(wdm, name):
+    if name not in _CFGS:
+        _CFGS[name] = wdm.configs.build(name)
+    return _CFGS[name]
+
+
+@pytest.mark.parametrize("name", ["c1", "c2"])
 @pytest.mark.parametrize("prec,tol", [("f64", 1e-11), ("f32", 1e-5)])
-def test_c2_steps_reverse_to_psi0(wdm, nativelib, c2, prec, tol):
-    print(f"c2 384^3 {prec}: {NSTEPS} steps forward and back", flush=True)
-    with _plan(wdm, nativelib, c2, prec) as p:
-        dt, _ = wdm.configs.autoset_dt(p, c2["steps"])
+def test_steps_reverse_to_psi0(wdm, nativelib, name, prec, tol):
+    cfg = _cfg(wdm, name)
+    print(f"{name} {cfg['shape']} {prec}: {NSTEPS} steps forward and back", flush=True)
+    with _plan(wdm, nativelib, cfg, prec) as p:
+        dt, _ = wdm.configs.autoset_dt(p, cfg["steps"])
         psi0 = p.get_psi()
         p.step(NSTEPS, dt, 0.5 * dt, 0.0)
         moved = _rel_l2(p.get_psi(), psi0)
@@ -60,10 +67,12 @@ def test_c2_steps_reverse_to_psi0(wdm, nativelib, c2, prec, tol):
     assert err < tol, err
 
 
-def test_c2_mass_conserved_fp64(wdm, nativelib, c2):
-    print("c2 384^3 f64: mass over 10 steps", flush=True)
-    with _plan(wdm, nativelib, c2, "f64") as p:
-        dt, _ = wdm.configs.autoset_dt(p, c2["steps"])
+@pytest.mark.parametrize("name", ["c1", "c2"])
+def test_mass_conserved_fp64(wdm, nativelib, name):
+    cfg = _cfg(wdm, name)
+    print(f"{name} {cfg['shape']} f64: mass over 10 steps", flush=True)
+    with _plan(wdm, nativelib, cfg, "f64") as p:
+        dt, _ = wdm.configs.autoset_dt(p, cfg["steps"])
         m0 = p.reduce(1)["mass"]
         p.step(10, dt, 0.5 * dt, 0.0)
         m1 = p.reduce(1)["mass"]
@@ -71,14 +80,16 @@ def test_c2_mass_conserved_fp64(wdm, nativelib, c2):
     assert abs(m1 - m0) / m0 < 1e-12, (m0, m1)
 
 
-def test_c2_fp32_tracks_fp64_short_horizon(wdm, nativelib, c2):
-    print("c2 384^3: f32 against f64 after 5 steps", flush=True)
-    with _plan(wdm, nativelib, c2, "f32") as p32:
-        dt, _ = wdm.configs.autoset_dt(p32, c2["steps"])
+@pytest.mark.parametrize("name", ["c1", "c2"])
+def test_fp32_tracks_fp64_short_horizon(wdm, nativelib, name):
+    cfg = _cfg(wdm, name)
+    print(f"{name} {cfg['shape']}: f32 against f64 after 5 steps", flush=True)
+    with _plan(wdm, nativelib, cfg, "f32") as p32:
+        dt, _ = wdm.configs.autoset_dt(p32, cfg["steps"])
         psi0 = p32.get_psi(np.complex64)
         p32.step(5, dt, 0.5 * dt, 0.0)
         out32 = p32.get_psi()
-    with _plan(wdm, nativelib, c2, "f64") as p64:
+    with _plan(wdm, nativelib, cfg, "f64") as p64:
         p64.set_psi(psi0)
         p64.step(5, dt, 0.5 * dt, 0.0)
         out64 = p64.get_psi()
