@@ -12,6 +12,7 @@ workload (64 solitons, Hernquist host, self-gravity) and the 384^3 workload
 * FP32 against FP64 from the same (FP32-rounded) psi0 after 5 steps: relative
   L2 <= 1e-5, the FP32 precision form's bar for psi, at a horizon where the
   dynamics do not yet amplify rounding.
+* FP32 mass with the mass projection: drift <= 1e-9 over 1000 steps (384^3).
 
 The adjoint workload (128^3, 200 steps): gradient against central
 differences.  The batched workload (48 boxes): every box equals its single
@@ -96,6 +97,22 @@ def test_fp32_tracks_fp64_short_horizon(wdm, nativelib, name):
     err = _rel_l2(out32, out64)
     print(f"  rel L2 {err:.3e}", flush=True)
     assert err < 1e-5, err
+
+
+def test_c2_fp32_mass_projection_1000_steps(wdm, nativelib):
+    """The FP32 precision form's mass bar on the 384^3 workload: relative mass
+    drift <= 1e-9 over 1000 steps with the opt-in mass projection."""
+    cfg = _cfg(wdm, "c2")
+    print("c2 (384, 384, 384) f32: mass over 1000 steps, projection on", flush=True)
+    with _plan(wdm, nativelib, cfg, "f32") as p:
+        dt, _ = wdm.configs.autoset_dt(p, cfg["steps"])
+        p.set_mass_ledger(True)
+        m0 = p.reduce(1)["mass"]
+        p.step(1000, dt, 0.5 * dt, 0.0)
+        m1 = p.reduce(1)["mass"]
+    drift = abs(m1 - m0) / m0
+    print(f"  drift {drift:.3e}", flush=True)
+    assert drift <= 1e-9, drift
 
 
 def test_c4_gradient_matches_central_differences(wdm, nativelib):
