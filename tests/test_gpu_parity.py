@@ -71,6 +71,11 @@ def test_parallel_poisson_mirror_reference_sanity(wdm, nativelib):
     assert 1e-6 < np.abs(phi).max() < 100.0
     ref = O.parallel_poisson((1.0, 1.0, 1.0), (32, 32, 32), rho)
     assert rel_max(phi, ref) < 1e-12
+    # the plan is reused for the same grid, and freed on request
+    assert np.array_equal(wdm.kdk.parallel_poisson((1.0, 1.0, 1.0), (32, 32, 32), rho), phi)
+    assert len(wdm.kdk._poisson_plans) == 1
+    wdm.kdk.clear_poisson_plans()
+    assert not wdm.kdk._poisson_plans
 
 
 def _hernquist_V(M, a, c, G=1.0):

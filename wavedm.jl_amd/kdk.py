@@ -216,7 +216,16 @@ def _install_potential(plan: Plan, V, t, phi_b=None):
 
 
 # ---------------------------------------------------------------- solver API
+# parallel_poisson keeps one plan per (shape, box, precision), as the
+# reference's callers reuse one FFT plan per grid; clear_poisson_plans frees them.
 _poisson_plans: dict = {}
+
+
+def clear_poisson_plans():
+    """Destroy the plans parallel_poisson keeps (their device buffers are freed)."""
+    while _poisson_plans:
+        _, plan = _poisson_plans.popitem()
+        plan.close()
 
 
 def parallel_poisson(delta, N, rho, boundary="periodic", backend=None, gpu_force=False):
