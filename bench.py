@@ -199,18 +199,6 @@ def copy_calibration(mib, reps):
                       "gbs_read_plus_write": 2 * 4 * n / (ms * 1e-3) / 1e9}), flush=True)
 
 
-def access_probe():
-    """Runs lib/probe_access (built by csrc/build.py): the in-place bandwidth of
-    the column passes' lane layouts at 384^3 complex FP32 with no FFT
-    arithmetic, i.e. the ceiling those passes can reach.  Not part of the
-    benchmark line.  Started as a child process before this process touches
-    the GPU; its exit status is returned."""
-    exe = os.path.join(ROOT, "wavedm.jl_amd", "lib", "probe_access")
-    if not os.path.exists(exe):
-        raise SystemExit(f"{exe} missing: build it with wavedm.jl_amd/csrc/build.py probe")
-    return subprocess.run([exe]).returncode
-
-
 def adjoint_leg(W, steps=None):
     """Cost of the gradient on the adjoint workload (c4: 128^3, 16 solitons,
     Hernquist host, 200 FP32 steps, checkpoints every 20): wall time of the
@@ -321,15 +309,11 @@ def main():
     ap.add_argument("--skip-ensemble", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration)")
-    ap.add_argument("--access-probe", action="store_true",
-                    help="only run the access-pattern bandwidth probe (lib/probe_access)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this library runs on one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU slab decomposition is a design note, not implemented", file=sys.stderr)
         sys.exit(2)
-    if args.access_probe:
-        sys.exit(access_probe())
     if args.copy_calibration:
         copy_calibration(args.copy_calibration, max(args.steps, 1))
         return

@@ -117,6 +117,15 @@ int wdm_set_self_gravity(wdm_plan* plan, int on);
    grow with the number of steps.  Needs self-gravity and no absorber; not
    differentiated by the adjoint. */
 int wdm_set_mass_ledger(wdm_plan* plan, int on);
+/* The projection's state, for snapshots (no counterpart in the reference,
+   which keeps no exact-restart state).  wdm_get_mass_ledger: *on = 1 if the
+   projection is on; *m_ref = the reference mass (sum |psi|^2, no cell volume)
+   it holds the state to, or NaN while it is not seeded (it is seeded from psi
+   by the next kick or step after wdm_set_mass_ledger or a new psi).
+   wdm_set_mass_reference: seeds it with m_ref (> 0) instead, so that a run
+   restored from a snapshot continues as the saved run would have. */
+int wdm_get_mass_ledger(wdm_plan* plan, int* on, double* m_ref);
+int wdm_set_mass_reference(wdm_plan* plan, double m_ref);
 
 int wdm_set_psi(wdm_plan* plan, const void* host_psi, int dtype);
 int wdm_get_psi(wdm_plan* plan, void* host_psi, int dtype);
@@ -267,10 +276,14 @@ int wdm_acceleration(wdm_plan* plan, double* host_ax, double* host_ay, double* h
    src/io/save.jl:79-81).  wdm_density_accumulate adds |psi|^2 of the current
    psi to an FP64 device sum; wdm_density_average writes sum / count to a host
    array of nx*ny*nz doubles (host_out may be NULL to read only *count);
-   wdm_density_reset clears the sum. */
+   wdm_density_reset clears the sum.  wdm_density_sum reads the FP64 sum
+   itself and wdm_density_restore replaces it (count > 0) -- the exact state a
+   snapshot needs to continue the average (the reference restarts it). */
 int wdm_density_accumulate(wdm_plan* plan);
 int wdm_density_average(wdm_plan* plan, double* host_out, long long* count);
 int wdm_density_reset(wdm_plan* plan);
+int wdm_density_sum(wdm_plan* plan, double* host_sum, long long* count);
+int wdm_density_restore(wdm_plan* plan, const double* host_sum, long long count);
 
 /* Batched ensemble of independent plans (uneven grids; no counterpart in the
    reference, which steps one box at a time).  The plans stay owned by the

@@ -47,3 +47,18 @@ def test_version_and_argument_errors_without_device(wdm):
     assert lib.wdm_plan_create(ctypes.byref(h), 8, 8, 8, 1.0, 1.0, 1.0, 7, 0) == A.WDM_ERR_ARG
     assert lib.wdm_plan_create(None, 8, 8, 8, 1.0, 1.0, 1.0, 0, 0) == A.WDM_ERR_ARG
     assert lib.wdm_step(None, 1, 1.0, 0.5, 0.0, 0) == A.WDM_ERR_ARG
+
+
+def test_plan_create_rejects_arrays_past_32bit_offsets(wdm):
+    """The pass kernels form byte offsets in 32 bits: a plan whose psi (or half
+    spectrum) would reach 4 GiB is refused before any device call.  1024^3
+    has 2^30 cells (below the 2^31-cell limit) but 8 GiB of FP32 psi; 646^3
+    FP64 and 813^3 FP32 are the first cubes past 2^32 bytes."""
+    A = wdm._capi
+    lib = A.load()
+    h = ctypes.c_void_p()
+    for n, prec in ((1024, A.WDM_F32), (812, A.WDM_F64), (646, A.WDM_F64), (813, A.WDM_F32)):
+        rc = lib.wdm_plan_create(ctypes.byref(h), n, n, n, 1.0, 1.0, 1.0, prec, 0)
+        assert rc == A.WDM_ERR_UNSUPPORTED, (n, prec, rc)
+        assert b"4 GiB" in lib.wdm_last_error(None)
+        assert not h.value

@@ -134,3 +134,45 @@ def test_adjoint_error_paths(wdm, nativelib):
         assert e.value.code == wdm._capi.WDM_ERR_UNSUPPORTED
         g0, gp = p.adjoint_backward(g, dt_scale=False)
         assert np.all(g0 == 0) and np.isnan(gp[5])
+
+
+def test_adjoint_tape_refuses_changed_plan(wdm, nativelib):
+    """The backward sweep recomputes its segments with the plan's current
+    models and restores psi_N at the end: a parameter change, a new psi or a
+    step between the taped forward and the backward sweep is refused with
+    WDM_ERR_STATE instead of returning the gradient of another trajectory.
+    A second backward on an unchanged plan repeats the first exactly."""
+    pr = _problem((16, 12, 8), False, 0.0)
+    psi0 = _psi0(pr.shape)
+    rng = np.random.default_rng(5)
+    g = rng.normal(size=pr.shape) + 1j * rng.normal(size=pr.shape)
+    with _plan(wdm, nativelib, pr, "f64") as p:
+        p.set_psi(psi0)
+        p.adjoint_forward(4, pr.dt, ckpt_every=2)
+        g0a, gpa = p.adjoint_backward(g)
+        g0b, gpb = p.adjoint_backward(g)   # same tape, unchanged plan
+        assert np.array_equal(g0a, g0b) and np.array_equal(gpa, gpb)
+        for change in ("hernquist", "coeffs", "set_psi", "step"):
+            p.set_psi(psi0)
+            p.adjoint_forward(4, pr.dt, ckpt_every=2)
+            if change == "hernquist":
+                p.set_hernquist(HQ[0] * 1.1, HQ[1], HQ[2:])
+            elif change == "coeffs":
+                p.set_coeffs(1.0, FOUR_PI_G * 1.01)
+            elif change == "set_psi":
+                p.set_psi(psi0 * 1.01)
+            else:
+                p.step(1, pr.dt)
+            with pytest.raises(wdm.WaveDMError) as e:
+                p.adjoint_backward(g)
+            assert e.value.code == wdm._capi.WDM_ERR_STATE, change
+            assert "tape" in str(e.value), change
+            # restore the configuration for the next case
+            p.clear_potential_models()
+            p.set_hernquist(HQ[0], HQ[1], HQ[2:])
+            p.set_coeffs(1.0, FOUR_PI_G)
+        # after re-taping on the restored plan the gradient is the first one again
+        p.set_psi(psi0)
+        p.adjoint_forward(4, pr.dt, ckpt_every=2)
+        g0c, gpc = p.adjoint_backward(g)
+        assert np.array_equal(g0a, g0c) and np.array_equal(gpa, gpc)

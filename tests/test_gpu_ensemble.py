@@ -86,7 +86,8 @@ def test_graph_refuses_time_dependent_potential(wdm, nativelib):
         p.close()
 
 
-VARIANT_SHAPES = [(64, 8, 8), (8, 128, 8), (8, 8, 192), (256, 8, 8), (384, 8, 8), (32, 16, 8)]
+VARIANT_SHAPES = [(64, 8, 8), (8, 128, 8), (8, 8, 192), (256, 8, 8), (384, 8, 8), (32, 16, 8),
+                  (8, 384, 8), (8, 8, 384), (8, 256, 8), (8, 8, 256), (8, 160, 8), (8, 8, 160)]
 
 
 def _available(wdm, lib, shape, prec):

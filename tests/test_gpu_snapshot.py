@@ -64,3 +64,60 @@ def test_density_average_matches_host_mean(wdm, nativelib):
     assert n == 5
     ref = acc / 5
     assert np.abs(avg - ref).max() <= 1e-13 * ref.max()
+
+
+def test_restart_with_mass_projection_and_average(wdm, nativelib, tmp_path):
+    """FP32 with the mass projection on and a running density average: the
+    snapshot carries the projection's reference mass and the FP64 density
+    sum, so the restored run's second call equals the saved run's second call
+    bit for bit, and the average continues over both calls."""
+    shape, dt = (32, 24, 16), 2e-3
+    psi0 = _psi(shape)
+
+    def first_call(p):
+        p.set_hernquist(8.0, 0.6, (0.2, -0.1, 0.0))
+        p.set_psi(psi0)
+        p.set_mass_ledger(True)
+        p.step(3, dt)
+        p.density_accumulate()
+
+    with wdm.Plan(shape, X, "f32", four_pi_G=3.0, lib=nativelib) as a:
+        first_call(a)
+        m_ref = a.mass_ledger_state()
+        a.step(4, dt, t0=3 * dt)
+        a.density_accumulate()
+        ref = a.get_psi()
+        ref_avg, ref_n = a.density_average()
+    with wdm.Plan(shape, X, "f32", four_pi_G=3.0, lib=nativelib) as b:
+        first_call(b)
+        wdm.snapshot.save(tmp_path / "s.npz", b, t=3 * dt, step=3, dt=dt, average=True)
+    snap = wdm.snapshot.load(tmp_path / "s.npz")
+    assert snap["mass_ledger"] is True and snap["mass_reference"] == m_ref[1] and m_ref[0]
+    assert snap["average_count"] == 1 and snap["psi2_sum"].dtype == np.float64
+    with wdm.snapshot.plan_from(snap, lib=nativelib) as c:
+        assert c.mass_ledger_state() == (True, snap["mass_reference"])
+        c.set_hernquist(8.0, 0.6, (0.2, -0.1, 0.0))
+        c.step(4, snap["dt"], t0=snap["t"])
+        c.density_accumulate()
+        out = c.get_psi()
+        avg, n = c.density_average()
+    assert np.array_equal(out, ref)
+    assert n == ref_n == 2 and np.array_equal(avg, ref_avg)
+
+
+def test_mass_reference_errors(wdm, nativelib):
+    with wdm.Plan((16, 12, 8), X, "f32", lib=nativelib) as p:
+        p.set_psi(_psi((16, 12, 8)))
+        assert p.mass_ledger_state() == (False, None)
+        with pytest.raises(wdm.WaveDMError) as e:
+            p.set_mass_reference(1.0)
+        assert e.value.code == wdm._capi.WDM_ERR_STATE
+        p.set_mass_ledger(True)
+        assert p.mass_ledger_state() == (True, None)   # seeded by the next step
+        with pytest.raises(wdm.WaveDMError):
+            p.set_mass_reference(-1.0)
+        p.step(1, 1e-3)
+        on, m = p.mass_ledger_state()
+        assert on and m > 0
+        p.set_mass_reference(2.0 * m)
+        assert p.mass_ledger_state() == (True, 2.0 * m)

@@ -37,3 +37,31 @@ def test_compute_timestep_host_arrays(wdm):
     assert len(t) == Nt and t[0] == 0.0 and t[-1] == Tmax
     # autoset off keeps the caller's Nt
     assert K.compute_timestep(dx, phi, 0.0, psi, Tmax, 10, False, 0.5)[2] == 10
+
+
+def test_baryon_mesh_potential_coefficient(wdm):
+    """An explicit four_pi_G = 0 gives a zero potential (the C ABI's scale 0
+    would mean the plan's coefficient); None passes the plan's coefficient
+    explicitly, a value passes through."""
+    K = wdm.kdk
+
+    class FakePlan:
+        shape = (2, 3, 4)
+        four_pi_G = 7.5
+
+        def __init__(self):
+            self.scales = []
+
+        def poisson(self, rho, scale=0.0):
+            self.scales.append(scale)
+            return np.full(self.shape, scale)
+
+    p = FakePlan()
+    rho = np.ones(p.shape)
+    out = K.baryon_mesh_potential(p, rho, four_pi_G=0.0)
+    assert out.shape == p.shape and not np.any(out) and p.scales == []
+    K.baryon_mesh_potential(p, rho)
+    K.baryon_mesh_potential(p, rho, four_pi_G=2.0)
+    assert p.scales == [7.5, 2.0]
+    with pytest.raises(ValueError, match="shape"):
+        K.baryon_mesh_potential(p, np.ones((2, 2, 2)))

@@ -152,6 +152,46 @@ def test_poisson_laplacian_inverts_spectral_laplacian():
     np.testing.assert_allclose(lap, rho - rho.mean(), atol=1e-12)
 
 
+@pytest.mark.parametrize("shape,X", [((8, 12, 16), (3.0, 2.5, 2.0)), ((16, 16, 10), (10.0, 10.0, 4.0))])
+def test_poisson_lattice_is_the_drift_operators_laplacian(shape, X):
+    """Ties the oracle's Poisson lattice to reference text rather than to its own
+    k formula.  The Laplacian is read off the reference's drift operator,
+    written out here from src/solver/fft_operators.jl:9-13 (f on
+    LinRange(-N/(4X), N/(4X) - 1/(2X), N), Lap = sum (2 pi i f)^2,
+    linear_phase = fftshift(exp(i Lap dt/2)); dt small enough that the phase
+    does not wrap, so Lap = 2 arg(linear_phase)/dt).  The oracle's potential of
+    src/solver/gravity.jl:13-23 (4 pi * parallel_poisson(Delta, [N-1...], rho -
+    mean)) must then satisfy Lap * F(Phi) = 4 pi F(rho - mean) at every k != 0
+    on the reference grid (Delta = 2X/(N-1), even N).  A Poisson period of
+    N*Delta instead of (N-1)*Delta = 2X fails by a factor ((N-1)/N)^2."""
+    lap_axes = []
+    for n, Xa in zip(shape, X):
+        f = O.linrange(-n / 4.0 / Xa, n / 4.0 / Xa - 1.0 / 2.0 / Xa, n)  # LinRange (pinned above)
+        lap_axes.append(np.roll(f, n // 2))  # Julia fftshift = circshift by n div 2
+    fx, fy, fz = lap_axes
+    Lap = ((2j * np.pi * fx[:, None, None]) ** 2 + (2j * np.pi * fy[None, :, None]) ** 2
+           + (2j * np.pi * fz[None, None, :]) ** 2).real
+    dt = 1e-3 / np.abs(Lap).max()
+    linear_phase = np.exp(1j * Lap * dt / 2)
+    Lap_read = 2.0 * np.angle(linear_phase) / dt
+    np.testing.assert_allclose(Lap_read, Lap, rtol=1e-9, atol=1e-12 * np.abs(Lap).max())
+    # the oracle's drift operator is the same array (its own restatement)
+    np.testing.assert_allclose(O.setup_fft_operators(*X, *shape, dt), linear_phase, atol=1e-14)
+    x, y, z, delta, _ = O.setup_grid(*X, *shape)
+    rng = np.random.default_rng(11)
+    psi = rng.standard_normal(shape) + 1j * rng.standard_normal(shape) + 1.5
+    phi = O.compute_gravitational_potential(psi, delta)
+    rho = np.abs(psi) ** 2
+    lhs = Lap_read * np.fft.fftn(phi)
+    rhs = 4.0 * np.pi * np.fft.fftn(rho - rho.mean())
+    nz = np.ones(shape, dtype=bool)
+    nz[0, 0, 0] = False  # FFT order: k = 0 first on every axis
+    assert np.abs(Lap_read[0, 0, 0]) <= 1e-12 * np.abs(Lap).max()
+    assert np.all(np.abs(Lap_read[nz]) > 1e-6 * np.abs(Lap).max())
+    np.testing.assert_allclose(lhs[nz], rhs[nz], rtol=1e-9, atol=1e-9 * np.abs(rhs).max())
+    assert abs(np.fft.fftn(phi)[0, 0, 0]) < 1e-9 * np.abs(rhs).max()
+
+
 # ---------------------------------------------------------------- kick
 def test_kick_preserves_modulus_and_sets_phase():
     rng = np.random.default_rng(7)

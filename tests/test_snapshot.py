@@ -25,8 +25,11 @@ class _FakePlan:
     def get_phi(self):
         return self.phi
 
-    def density_average(self):
-        return self.avg, 4
+    def density_sum(self):
+        return self.avg * 4.0, 4
+
+    def mass_ledger_state(self):
+        return (self.precision == "f32"), (123.5 if self.precision == "f32" else None)
 
 
 @pytest.mark.parametrize("precision", ["f64", "f32"])
@@ -38,8 +41,11 @@ def test_snapshot_round_trip(tmp_path, precision):
     assert z["format"] == S.FORMAT and z["t"] == 1.25 and z["step"] == 40 and z["dt"] == 0.03125
     assert tuple(z["shape"]) == p.shape and z["precision"] == precision and z["average_count"] == 4
     assert z["psi"].dtype == p.psi.dtype and np.array_equal(z["psi"], p.psi) and z["psi"].flags.f_contiguous
-    assert np.array_equal(z["phi_all"], p.phi) and np.array_equal(z["psi2_averaged"], p.avg)
+    assert np.array_equal(z["phi_all"], p.phi) and np.array_equal(z["psi2_sum"], p.avg * 4.0)
+    assert np.array_equal(z["psi2_averaged"], (p.avg * 4.0) / 4.0)
     assert z["meta"] == {"run": "x"} and z["hbar_over_m"] == 0.5 and z["four_pi_G"] == 7.0
+    assert z["mass_ledger"] == (precision == "f32")
+    assert z["mass_reference"] == (123.5 if precision == "f32" else None)
 
 
 def test_snapshot_rejects_foreign_files(tmp_path):
@@ -64,3 +70,21 @@ def test_prop_table_columns_and_values(tmp_path):
     assert rows[0][0] == 0.5 and rows[0][1] == 0.0 and rows[0][2] == 2.0 and rows[1][2] == 4.0
     assert rows[1][cols.index("MomentumZ")] == diag["MomentumZ"] + 1
     assert S.PropTable(virial=False).columns == S.PROP_COLUMNS
+
+
+def test_prop_table_total_mass_lags_one_row():
+    """The reference pushes total_halo_mass before recomputing it from the new
+    state (src/simulation_main.jl:557 vs :652; seeded with the IC mass at
+    :349), so TotalMass row k is the mass of row k-1's state, and an infinite
+    mass is recorded as 0."""
+    cols = S.PROP_COLUMNS[1:]
+    states = [dict({c: 1.0 for c in cols}, TotalMass=m) for m in (10.0, float("inf"), 12.0)]
+    tab = S.PropTable(virial=False, initial_mass=9.0, scale={"TotalMass": 2.0})
+    for k, d in enumerate(states):
+        tab.append(0.1 * k, d)
+    assert [r["TotalMass"] for r in tab.rows] == [18.0, 20.0, 0.0]
+    # without the IC mass the first row carries its own state's mass
+    tab = S.PropTable(virial=False)
+    for k, d in enumerate(states):
+        tab.append(0.1 * k, d)
+    assert [r["TotalMass"] for r in tab.rows] == [10.0, 10.0, 0.0]

@@ -88,7 +88,11 @@ SIGNATURES = {
     "wdm_density_accumulate": (_I, [_P]),
     "wdm_density_average": (_I, [_P, _DP, C.POINTER(C.c_longlong)]),
     "wdm_density_reset": (_I, [_P]),
+    "wdm_density_sum": (_I, [_P, _DP, C.POINTER(C.c_longlong)]),
+    "wdm_density_restore": (_I, [_P, _DP, C.c_longlong]),
     "wdm_set_mass_ledger": (_I, [_P, _I]),
+    "wdm_get_mass_ledger": (_I, [_P, C.POINTER(_I), _DP]),
+    "wdm_set_mass_reference": (_I, [_P, _D]),
     "wdm_ic_velocity": (_I, [_P, _DP, _DP, _DP, _DP, _I]),
     "wdm_step_timed_mask": (_I, [_P, _I, C.c_double, C.c_double, C.c_double, C.c_uint, _DP, _I, _DP]),
     "wdm_cfl": (_I, [_P, _D, _DP]),

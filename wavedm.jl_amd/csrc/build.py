@@ -1,15 +1,12 @@
 """Build driver for the wavedm native libraries.
 
-    python build.py [hip|emu|probe|all] [--jobs N] [--incremental]
+    python build.py [hip|emu|all] [--jobs N] [--incremental]
 
 * ``hip``: lib/libwavedm_hip.so, the product, HIP for gfx950 (hipcc).
 * ``emu``: lib/libwavedm_cpuemu.so, a CPU fiber-emulation build of the same
   kernel source compiled with g++ and AddressSanitizer.  Debug/test only: it
   exists so the host-side tests can bounds-check every global and LDS access
   of the kernels; nothing in the package loads it.
-* ``probe``: lib/probe_access, a standalone executable that measures the
-  bandwidth ceiling of the column passes' access patterns (bench.py
-  --access-probe runs it).
 
 One translation unit per supported axis length (the WDM_FOR_EACH_SIZE table in
 wdm_ops.hpp) so the kernel instantiations compile in parallel.
@@ -119,29 +116,15 @@ def build(kind="hip", jobs=None, incremental=False, verbose=True, defines=(), ou
     return target
 
 
-def build_probe(verbose=True):
-    target = os.path.join(LIB, "probe_access")
-    os.makedirs(LIB, exist_ok=True)
-    _run([HIPCC] + [f for f in HIP_FLAGS if f != "-fPIC"] + [os.path.join(HERE, "probe_access.hip"),
-                                                              "-o", target + ".tmp"])
-    shutil.move(target + ".tmp", target)
-    if verbose:
-        print(f"[build:probe] -> {target}", flush=True)
-    return target
-
-
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", nargs="?", default="hip", choices=["hip", "emu", "probe", "all"])
+    ap.add_argument("what", nargs="?", default="hip", choices=["hip", "emu", "all"])
     ap.add_argument("--jobs", type=int, default=None)
     ap.add_argument("--incremental", action="store_true")
     a = ap.parse_args(argv)
-    kinds = ["hip", "emu", "probe"] if a.what == "all" else [a.what]
+    kinds = ["hip", "emu"] if a.what == "all" else [a.what]
     for k in kinds:
-        if k == "probe":
-            build_probe()
-        else:
-            build(k, a.jobs, a.incremental)
+        build(k, a.jobs, a.incremental)
 
 
 if __name__ == "__main__":

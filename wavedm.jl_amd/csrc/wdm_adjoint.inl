@@ -143,6 +143,12 @@ void k_adj_zero(double* __restrict__ acc, int n) {
 // ------------------------------------------------------------------ host side
 struct AdjState {
     bool ready = false;
+    // the plan's configuration generation and psi epoch the tape belongs to:
+    // the backward sweep recomputes segments with the plan's current models
+    // and coefficients and ends by restoring psi_N, so either changing in
+    // between would give the gradient of another trajectory
+    unsigned gen = 0;
+    unsigned long long epoch = 0;
     int nsteps = 0, K = 0, nckpt = 0;
     double dt = 0.0, dt_kick = 0.0, t0 = 0.0;
     DBuf ckpt;   // (nckpt + 1) psi: psi_{kK}, k < nckpt, then psi_N
@@ -196,6 +202,8 @@ void adj_forward(wdm_plan* p, int nsteps, double dt, double dt_kick, double t0, 
                              hipMemcpyDeviceToDevice, p->stream),
               "final state");
     hip_check(hipStreamSynchronize(p->stream), "adjoint forward");
+    A.gen = p->gen;
+    A.epoch = p->psi_epoch;
     A.ready = true;
 }
 
@@ -245,6 +253,12 @@ void adj_backward(wdm_plan* p, const void* grad, int dtype, void* grad0, double*
     typedef typename TwEntry<T>::type TwE;
     AdjState& A = adj_of(p);
     if (!A.ready) fail(WDM_ERR_STATE, "wdm_adjoint_backward: no taped forward run (call wdm_adjoint_forward)");
+    if (A.gen != p->gen)
+        fail(WDM_ERR_STATE, "wdm_adjoint_backward: the plan's potential models, coefficients, border, gravity or "
+                            "mass projection changed since wdm_adjoint_forward (tape no longer matches)");
+    if (A.epoch != p->psi_epoch)
+        fail(WDM_ERR_STATE, "wdm_adjoint_backward: psi was set or stepped since wdm_adjoint_forward (tape no "
+                            "longer matches)");
     const bool want_dt = (flags & WDM_ADJ_DT_SCALE) != 0;
     if (want_dt && (p->vflags & V_QUADRUPOLE) && p->quad[9] != 0.0)
         fail(WDM_ERR_UNSUPPORTED, "wdm_adjoint_backward: dt-scale gradient needs a time-independent potential");
@@ -359,6 +373,7 @@ void adj_backward(wdm_plan* p, const void* grad, int dtype, void* grad0, double*
     p->kspace = false;
     p->phi_valid = false;
     p->mass_valid = false;
+    A.epoch = p->psi_epoch;  // psi_N restored: the tape still matches (a second backward may follow)
     if (grad0) {
         // download g_0 through the same conversion as wdm_get_psi
         cx<T>* keep = p->psi.as<cx<T>>();

@@ -471,6 +471,30 @@ int wdm_density_reset(wdm_plan* p) {
     return WDM_OK;
 }
 
+int wdm_density_sum(wdm_plan* p, double* host_sum, long long* count) {
+    if (!p) return fail_code(nullptr, WDM_ERR_ARG, "plan is NULL");
+    return guarded(p, [&] {
+        if (count) *count = p->rho_avg_count;
+        if (!host_sum) return;
+        if (p->rho_avg_count == 0) fail(WDM_ERR_STATE, "wdm_density_sum: nothing accumulated");
+        hip_check(hipMemcpyAsync(host_sum, p->rho_avg.as<double>(), sizeof(double) * p->ncell, hipMemcpyDeviceToHost,
+                                 p->stream),
+                  "density sum copy");
+        hip_check(hipStreamSynchronize(p->stream), "density sum sync");
+    });
+}
+
+int wdm_density_restore(wdm_plan* p, const double* host_sum, long long count) {
+    if (!p || !host_sum || count <= 0) return fail_code(p, WDM_ERR_ARG, "wdm_density_restore: bad argument");
+    return guarded(p, [&] {
+        double* acc = p->rho_avg.get<double>(p->ncell);
+        hip_check(hipMemcpyAsync(acc, host_sum, sizeof(double) * p->ncell, hipMemcpyHostToDevice, p->stream),
+                  "density restore copy");
+        hip_check(hipStreamSynchronize(p->stream), "density restore sync");
+        p->rho_avg_count = count;
+    });
+}
+
 int wdm_diagnostics(wdm_plan* p, double* out, int nout) {
     if (!p || !out || nout <= 0) return fail_code(p, WDM_ERR_ARG, "wdm_diagnostics: bad argument");
     return guarded(p, [&] { DISPATCH(p, diagnostics, p, out, nout); });

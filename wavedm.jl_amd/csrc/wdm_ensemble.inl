@@ -153,6 +153,7 @@ void ens_issue(wdm_ensemble* e, const double* dt, const double* dtk, double t0, 
 
 void ens_step(wdm_ensemble* e, int nsteps, const double* dt, const double* dtk, double t0, bool use_graph) {
     const size_t nm = e->m.size();
+    for (wdm_plan* p : e->m) ++p->psi_epoch;  // graph replays step psi without passing through do_step
     ens_prepare(e, dt);
     for (wdm_plan* p : e->m) hip_check(hipStreamSynchronize(p->stream), "member sync");
     if (use_graph)

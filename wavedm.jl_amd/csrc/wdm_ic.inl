@@ -117,6 +117,7 @@ void ic_velocity(wdm_plan* p, const double* rho, const double* const v[3], int r
                p->psi.as<cx<T>>(), n, 1.0 / p->hbar_over_m);
     hip_check(hipGetLastError(), "ic assemble");
     hip_check(hipStreamSynchronize(p->stream), "ic sync");
+    ++p->psi_epoch;
     p->kspace = false;
     p->phi_valid = false;
     p->mass_valid = false;

@@ -192,7 +192,9 @@ WDM_FI double eval_V_line(const VModel& vm, const LineV& l, double x, unsigned i
 }
 
 // ------------------------------------------------------------------ phases
-// Reduce theta to r in [-pi/4, pi/4] and quadrant q (theta = q*pi/2 + r), FP64.
+// Reduce theta to r in [-pi/4, pi/4] and quadrant q (theta = q*pi/2 + r), FP64:
+// Cody and Waite's reduction with pi/2 split into two constants, each
+// subtracted by a fused multiply-add so n*hi and n*lo enter unrounded.
 WDM_FI double reduce_quadrant(double theta, int& q) {
     constexpr double kTwoOverPi = 0.63661977236758134307553505349006;
     constexpr double kPio2Hi = 1.5707963267948965579989817342721;

@@ -68,7 +68,7 @@ WDM_FI void fwd2(cx<T>* v, cx<T> (*X)[F::N2], int j, cx<T>* __restrict__ xb,
             for (int b = 0; b < N2; ++b) {
                 const int k1 = i * N2 + b;
                 if (k1 > 0)
-                    v[k1] = mul_tab<DIR, P>(v[k1], tw[j * k1], twiddle_may_hit_half(F::N, k1, 0, N2));
+                    v[k1] = mul_tab<DIR, P>(v[k1], tw[j * k1]);
             }
         }
         cx<T> in[N2];
@@ -93,7 +93,7 @@ WDM_FI void inv2(cx<T> (*X)[F::N2], cx<T>* v, int j, cx<T>* __restrict__ xb,
             const int q = i * N2 + j;
             WDM_UNROLL
             for (int kp = 1; kp < N2; ++kp)
-                X[i][kp] = mul_tab<DIR, P>(X[i][kp], tw[q * kp], twiddle_may_hit_half(F::N, kp, i * N2 * kp, N2));
+                X[i][kp] = mul_tab<DIR, P>(X[i][kp], tw[q * kp]);
         }
     }
     WDM_UNROLL

@@ -237,45 +237,35 @@ template <class T> struct TwEntry;
 template <> struct TwEntry<double> { typedef cx<double> type; };
 template <> struct TwEntry<float> { typedef TwF type; };
 
-// a * ((ch+cl) + i (sh+sl)) with the operation order chosen per value so that
-// a power-of-two hi goes in the last fma (see is_pow2f).  Lane-varying
-// constants need the selects; compile-time ones fold them away.
-WDM_FI cx<float> cmul_hl(cx<float> a, float ch, float cl, float sh, float sl) {
-    const bool snice = is_pow2f(sh), cnice = is_pow2f(ch);
-    float t = fmaT(-a.y, sl, a.x * cl);
-    t = fmaT(snice ? a.x : -a.y, snice ? ch : sh, t);
-    t = fmaT(snice ? -a.y : a.x, snice ? sh : ch, t);
-    float u = fmaT(a.y, cl, a.x * sl);
-    u = fmaT(cnice ? a.x : a.y, cnice ? sh : ch, u);
-    u = fmaT(cnice ? a.y : a.x, cnice ? ch : sh, u);
-    return mk(t, u);
-}
-
-// Can exp(-2 pi i (A j + B)/N), j in [0, NJ), fall on a multiple of 30 deg
-// that is not a multiple of 90 deg (a component of exactly +-0.5)?  Called
-// with compile-time arguments, so it folds to a constant.
-WDM_HDFI bool twiddle_may_hit_half(int N, int A, int B, int NJ) {
-    for (int j = 0; j < NJ; ++j) {
-        const int e = ((A * j + B) % N + N) % N;
-        if ((12 * e) % N == 0 && (4 * e) % N != 0) return true;
+// Runtime table entries are split by split_hl_table: a "hi" part that would
+// be a power of two (1, 0.5 of the angles at multiples of 30 and 90 degrees)
+// is moved one ulp toward zero and the exact remainder goes into "lo".  Every
+// hi * x product is then inexact for general x, so one fixed fma order is
+// safe: the corrections ride on inexact products and the final fma rounds the
+// whole sum once (see is_pow2f).  No per-lane operand selects are needed.
+inline HL split_hl_table(double v) {
+    float h = float(v);
+    if (is_pow2f(h)) {
+        unsigned b = __builtin_bit_cast(unsigned, h);
+        h = __builtin_bit_cast(float, b - 1u);  // same sign, one ulp smaller magnitude
     }
-    return false;
+    return {h, float(v - double(h))};
 }
 
 // x * (table entry)^DIR-conjugated: forward (DIR<0) uses exp(-i theta).
 template <int DIR, class P>
-WDM_FI cx<double> mul_tab(cx<double> a, const cx<double>& w, bool = false) {
+WDM_FI cx<double> mul_tab(cx<double> a, const cx<double>& w) {
     const double s = DIR > 0 ? w.y : -w.y;
     return mk(fmaT(a.x, w.x, -(a.y * s)), fmaT(a.x, s, a.y * w.x));
 }
 template <int DIR, class P>
-WDM_FI cx<float> mul_tab(cx<float> a, const TwF& w, bool robust = false) {
+WDM_FI cx<float> mul_tab(cx<float> a, const TwF& w) {
     const float sh = DIR > 0 ? w.sh : -w.sh;
     if constexpr (std::is_same<P, Plain>::value) {
         return mk(fmaT(a.x, w.ch, -(a.y * sh)), fmaT(a.x, sh, a.y * w.ch));
     } else {
+        // entries from split_hl_table: no hi part is a power of two
         const float sl = DIR > 0 ? w.sl : -w.sl;
-        if (robust) return cmul_hl(a, w.ch, w.cl, sh, sl);
         float t = a.x * w.cl;
         t = fmaT(-a.y, sl, t);
         t = fmaT(-a.y, sh, t);
@@ -288,8 +278,10 @@ WDM_FI cx<float> mul_tab(cx<float> a, const TwF& w, bool robust = false) {
     }
 }
 
-// Multiply by an FP64 complex factor (c, s) computed at run time, in the data
-// precision: FP32 compensated splits it into hi+lo first.
+// Multiply by an FP64 complex factor (c, s) computed at run time.  FP32
+// compensated: the data are widened and the product is formed in FP64, so
+// each component is rounded to FP32 once from a result accurate to ~1e-16
+// (unbiased, whatever the factor's value).
 template <class P>
 WDM_FI cx<double> mul_dyn(cx<double> a, double c, double s) {
     return mk(fmaT(a.x, c, -(a.y * s)), fmaT(a.x, s, a.y * c));
@@ -300,8 +292,8 @@ WDM_FI cx<float> mul_dyn(cx<float> a, double c, double s) {
         const float cc = float(c), ss = float(s);
         return mk(fmaT(a.x, cc, -(a.y * ss)), fmaT(a.x, ss, a.y * cc));
     } else {
-        const HL cc = split_hl(c), ss = split_hl(s);
-        return cmul_hl(a, cc.h, cc.l, ss.h, ss.l);
+        const double x = double(a.x), y = double(a.y);
+        return mk(float(fmaT(x, c, -(y * s))), float(fmaT(x, s, y * c)));
     }
 }
 

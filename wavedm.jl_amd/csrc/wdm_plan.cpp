@@ -151,6 +151,7 @@ struct wdm_plan {
     AdjState* adj = nullptr;  // adjoint tape (wdm_adjoint_*), created on first use
     void* zspec_out = nullptr; // when set, the fused z pass also stores the 3-D spectrum here
     unsigned gen = 0;          // bumped by every change a captured step would not see
+    unsigned long long psi_epoch = 0;  // bumped whenever psi is replaced or stepped
     std::string err;
 };
 
@@ -194,7 +195,7 @@ void build_twiddles(AxisState& a) {
         if constexpr (std::is_same<T, double>::value) {
             h[j] = mk(v.c, v.s);
         } else {
-            const HL c = split_hl(v.c), s = split_hl(v.s);
+            const HL c = split_hl_table(v.c), s = split_hl_table(v.s);
             TwF w;
             w.ch = c.h;
             w.cl = c.l;
@@ -474,6 +475,7 @@ void x_inv_close(wdm_plan* p) {
 template <class T>
 void do_kick(wdm_plan* p, double dt_kick, double t) {
     if (p->kspace) fail(WDM_ERR_STATE, "wdm_kick: psi is in k-space (call wdm_drift first)");
+    ++p->psi_epoch;
     ledger_begin<T>(p);
     ensure_phi<T>(p);
     x_kick<T>(p, true, dt_kick, t);
@@ -485,6 +487,7 @@ void do_kick(wdm_plan* p, double dt_kick, double t) {
 template <class T>
 void do_drift(wdm_plan* p, double dt) {
     if (!p->kspace) fail(WDM_ERR_STATE, "wdm_drift: psi is not in k-space (call wdm_kick first)");
+    ++p->psi_epoch;
     z_pass<T>(p, COL_MI, dt);
     y_pass<T>(p, COL_INV);
     x_inv_close<T>(p);
@@ -495,6 +498,7 @@ template <class T>
 void do_step(wdm_plan* p, int nsteps, double dt, double dt_kick, double t0, int split) {
     if (p->kspace) fail(WDM_ERR_STATE, "wdm_step: psi is in k-space (call wdm_drift first)");
     if (nsteps <= 0) return;
+    ++p->psi_epoch;
     ledger_begin<T>(p);
     ensure_phi<T>(p);
     ensure_drift(p, dt);
@@ -703,8 +707,12 @@ void k_ledger_update(const double* __restrict__ partials, int n, double* __restr
 // End of a step sequence: psi *= f with unbiased stochastic rounding (the
 // product is formed in FP64 and rounded to the neighbour above or below with
 // probability given by the remainder, from a hash of the element index), so a
-// correction far below one FP32 ulp still moves the mass to M_ref on average;
-// Phi_hat *= f^2 with ordinary rounding.  FP64 plans round to nearest.
+// correction far below one FP32 ulp still moves the mass to M_ref on average.
+// FP64 plans round to nearest.  Phi_hat is not rescaled: the next call derives
+// it from the returned psi, so the state between calls is psi alone (a run
+// restored from psi continues exactly).
+// The index hash is the "lowbias32" integer hash (Chris Wellons'
+// hash-prospector constants).
 WDM_FI unsigned ledger_hash(unsigned x) {
     x ^= x >> 16;
     x *= 0x7feb352du;
@@ -737,14 +745,10 @@ WDM_FI cx<double> ledger_scale(cx<double> a, double f, size_t) { return a * f; }
 
 template <class T>
 WDM_KERNEL(256)
-void k_ledger_apply(cx<T>* __restrict__ psi, size_t n, cx<T>* __restrict__ hat, size_t nh,
-                    const double* __restrict__ ledger) {
+void k_ledger_apply(cx<T>* __restrict__ psi, size_t n, const double* __restrict__ ledger) {
     const double f = ledger[1];
     for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += size_t(gridDim.x) * 256)
         psi[i] = ledger_scale(psi[i], f, i);
-    if (hat)
-        for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < nh; i += size_t(gridDim.x) * 256)
-            hat[i] = hat[i] * T(f * f);
 }
 
 WDM_KERNEL(64)
@@ -774,17 +778,18 @@ void ledger_update(wdm_plan* p) {
     hip_check(hipGetLastError(), "ledger update");
 }
 
-// Apply the pending scale to the state a call returns (psi, and Phi_hat when
-// it is valid), then clear it so that the next z pass does not apply it again.
+// Apply the pending scale to the psi a call returns, then clear it so that
+// the next z pass does not apply it again; Phi_hat (of the unscaled psi) is
+// dropped and recomputed from the returned psi by the next call.
 template <class T>
 void ledger_finish(wdm_plan* p) {
     if (!p->ledger) return;
-    const size_t nh = (p->self_gravity && p->phi_valid) ? size_t(p->ny) * size_t(p->nz) * size_t(p->hpitch) : 0;
     const unsigned nb = unsigned(std::min<size_t>((p->ncell + 255) / 256, 65536));
     WDM_LAUNCH((k_ledger_apply<T>), dim3(nb), dim3(256), p->stream, p->psi.as<cx<T>>(), p->ncell,
-               nh ? p->rhohat.as<cx<T>>() : (cx<T>*)nullptr, nh, (const double*)p->ledger_buf.as<double>());
+               (const double*)p->ledger_buf.as<double>());
     WDM_LAUNCH((k_ledger_set), dim3(1), dim3(64), p->stream, p->ledger_buf.as<double>(), 0.0, 0);
     hip_check(hipGetLastError(), "ledger finish");
+    p->phi_valid = false;
 }
 
 template <class T>
@@ -895,6 +900,7 @@ void ic_solitons(wdm_plan* p, const double* params, int nsol) {
                (const double*)d, nsol, 2.0 * p->X[0], 2.0 * p->X[1], 2.0 * p->X[2]);
     hip_check(hipGetLastError(), "ic kernel");
     hip_check(hipStreamSynchronize(p->stream), "ic sync");
+    ++p->psi_epoch;
     p->kspace = false;
     p->phi_valid = false;
     p->mass_valid = false;
@@ -934,6 +940,7 @@ void set_psi_dev(wdm_plan* p, const void* src, int dtype, bool host) {
         else convert_dev<T, float>(d, src, n, p->stream);
     }
     hip_check(hipStreamSynchronize(p->stream), "set_psi sync");
+    ++p->psi_epoch;
     p->kspace = false;
     p->phi_valid = false;
     p->mass_valid = false;
@@ -1122,6 +1129,15 @@ int wdm_plan_create(wdm_plan** out, int nx, int ny, int nz, double xmax, double 
         return fail_code(nullptr, WDM_ERR_ARG, "wdm_plan_create: precision must be WDM_F64 or WDM_F32");
     if (double(nx) * ny * nz >= 2147483648.0)
         return fail_code(nullptr, WDM_ERR_UNSUPPORTED, "wdm_plan_create: grid exceeds 2^31 cells");
+    {
+        // The pass kernels address psi and the half spectrum with 32-bit byte
+        // offsets from a uniform base (ldg/stg): both arrays must stay below 4 GiB.
+        const double elem = precision == WDM_F64 ? 16.0 : 8.0;
+        const double hp = double(((nx / 2 + 1) + WDM_HPITCH_ALIGN - 1) / WDM_HPITCH_ALIGN * WDM_HPITCH_ALIGN);
+        if (double(nx) * ny * nz * elem >= 4294967296.0 || hp * ny * nz * elem >= 4294967296.0)
+            return fail_code(nullptr, WDM_ERR_UNSUPPORTED,
+                             "wdm_plan_create: psi or the half spectrum would reach 4 GiB (32-bit byte offsets)");
+    }
     wdm_plan* p = new (std::nothrow) wdm_plan();
     if (!p) return fail_code(nullptr, WDM_ERR_OOM, "wdm_plan_create: out of host memory");
     p->nx = nx;
@@ -1218,6 +1234,34 @@ int wdm_set_mass_ledger(wdm_plan* p, int on) {
     p->ledger = on != 0;
     p->ledger_stale = true;
     return WDM_OK;
+}
+
+int wdm_get_mass_ledger(wdm_plan* p, int* on, double* m_ref) {
+    if (!p || !on || !m_ref) return fail_code(p, WDM_ERR_ARG, "wdm_get_mass_ledger: NULL argument");
+    *on = p->ledger ? 1 : 0;
+    *m_ref = NAN;
+    if (!p->ledger || p->ledger_stale) return WDM_OK;
+    return guarded(p, [&] {
+        double h[2];
+        hip_check(hipMemcpyAsync(h, p->ledger_buf.as<double>(), sizeof(h), hipMemcpyDeviceToHost, p->stream),
+                  "ledger read");
+        hip_check(hipStreamSynchronize(p->stream), "ledger read sync");
+        *m_ref = h[0];
+    });
+}
+
+int wdm_set_mass_reference(wdm_plan* p, double m_ref) {
+    if (!p) return fail_code(nullptr, WDM_ERR_ARG, "plan is NULL");
+    if (!p->ledger) return fail_code(p, WDM_ERR_STATE, "wdm_set_mass_reference: the mass projection is off");
+    if (!(m_ref > 0) || !std::isfinite(m_ref))
+        return fail_code(p, WDM_ERR_ARG, "wdm_set_mass_reference: m_ref must be finite and > 0");
+    if (p->kspace) return fail_code(p, WDM_ERR_STATE, "wdm_set_mass_reference: psi is in k-space");
+    return guarded(p, [&] {
+        WDM_LAUNCH((k_ledger_set), dim3(1), dim3(64), p->stream, p->ledger_buf.get<double>(2), m_ref, 1);
+        hip_check(hipGetLastError(), "ledger set");
+        hip_check(hipStreamSynchronize(p->stream), "ledger set sync");
+        p->ledger_stale = false;
+    });
 }
 
 int wdm_set_psi(wdm_plan* p, const void* host, int dtype) {

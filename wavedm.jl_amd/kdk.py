@@ -253,11 +253,17 @@ def baryon_mesh_potential(plan: Plan, rho_baryon, four_pi_G=None):
     solved on ``plan``'s device lattice (period 2 Xmax per axis, as the
     reference's N Delta) in the plan's precision; real part, host FP64.
     ``four_pi_G`` None takes the plan's coefficient (the reference's literal 4 pi
-    in code units).  The plan's psi is left as it was."""
+    in code units); an explicit 0 gives Phi_b = 0 (no solve).  The plan's psi
+    is left as it was."""
     rho = np.asarray(rho_baryon, dtype=np.float64)
     if rho.shape != plan.shape:
         raise ValueError(f"rho_baryon must have shape {plan.shape}, got {rho.shape}")
-    return plan.poisson(rho, scale=0.0 if four_pi_G is None else float(four_pi_G))
+    scale = plan.four_pi_G if four_pi_G is None else float(four_pi_G)
+    if scale == 0.0:
+        # the C ABI reads scale 0 as "the plan's coefficient": a zero
+        # coefficient is answered here
+        return np.zeros(plan.shape, dtype=np.float64)
+    return plan.poisson(rho, scale=scale)
 
 
 def compute_gravitational_potential(device_psi: Plan, config_gravity=None, grid=None, config_device=None):

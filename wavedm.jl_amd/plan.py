@@ -79,6 +79,19 @@ class Plan:
         needs self-gravity and no absorber)."""
         self._chk(self.lib.wdm_set_mass_ledger(self._h, 1 if on else 0))
 
+    def mass_ledger_state(self):
+        """(on, m_ref): whether the mass projection is on and the reference
+        mass it holds (sum |psi|^2 without the cell volume; None while not
+        seeded)."""
+        on = C.c_int(0)
+        m = C.c_double(0.0)
+        self._chk(self.lib.wdm_get_mass_ledger(self._h, C.byref(on), C.byref(m)))
+        return bool(on.value), (None if np.isnan(m.value) else float(m.value))
+
+    def set_mass_reference(self, m_ref):
+        """Seed the mass projection's reference mass (restart from a snapshot)."""
+        self._chk(self.lib.wdm_set_mass_reference(self._h, float(m_ref)))
+
     def set_border(self, border):
         if border is None:
             self._chk(self.lib.wdm_set_border(self._h, None, 1))
@@ -260,6 +273,25 @@ class Plan:
 
     def density_reset(self):
         self._chk(self.lib.wdm_density_reset(self._h))
+
+    def density_sum(self):
+        """(FP64 sum of |psi|^2 over the accumulated steps, count); (None, 0)
+        when nothing was accumulated."""
+        n = C.c_longlong(0)
+        self._chk(self.lib.wdm_density_sum(self._h, None, C.byref(n)))
+        if n.value == 0:
+            return None, 0
+        out = np.empty(self.ncell, dtype=np.float64)
+        self._chk(self.lib.wdm_density_sum(self._h, _dptr(out), C.byref(n)))
+        return out.reshape(self.shape, order="F"), int(n.value)
+
+    def density_restore(self, total, count):
+        """Replace the device sum and its count (continue a saved average)."""
+        t = np.asarray(total, dtype=np.float64)
+        if t.shape != self.shape:
+            raise ValueError(f"sum must have shape {self.shape}")
+        buf = np.ascontiguousarray(t.ravel(order="F"))
+        self._chk(self.lib.wdm_density_restore(self._h, _dptr(buf), int(count)))
 
     def cfl(self, kappa=0.0):
         out = np.zeros(3, dtype=np.float64)

@@ -12,6 +12,18 @@ grid, the coefficients, the time, the step counter and dt, so that
 for bit.  Potential models and the absorber are configuration, not state: the
 caller installs them again, as the reference rebuilds them from its config.
 
+Two pieces of device state beyond psi are saved so that this holds for every
+plan option:
+* the mass projection (wdm_set_mass_ledger): its on/off flag and the
+  reference mass it holds the state to (restored with wdm_set_mass_reference;
+  re-seeding it from the restored psi would move it by the last step's
+  rounding).  The projection applies its pending scale when a step call
+  returns, so "uninterrupted" means the same sequence of step calls: a run
+  saved between two calls continues exactly as the second call would have.
+* the time-averaged density: the FP64 device sum itself ("psi2_sum") and its
+  count, next to the reference's "psi2_averaged" = sum / count, so the
+  average continues where it stopped (the reference restarts it).
+
 ``PropTable`` is save_property_dataframe (src/io/save.jl:41-72): one row per
 step of the wdm_diagnostics values under the reference's Prop.csv column
 names and order, written as CSV.
@@ -20,6 +32,7 @@ from __future__ import annotations
 
 import csv
 import json
+import math
 
 import numpy as np
 
@@ -38,14 +51,17 @@ def save(path, plan, *, t, step, dt=None, phi_all=False, psi_last_t=None, averag
         arrays["psi_last_t"] = np.asarray(psi_last_t)
     avg_n = 0
     if average:
-        avg, avg_n = plan.density_average()
-        if avg is not None:
-            arrays["psi2_averaged"] = avg
+        total, avg_n = plan.density_sum()
+        if total is not None:
+            arrays["psi2_averaged"] = total / float(avg_n)
+            arrays["psi2_sum"] = total
+    ledger_on, m_ref = plan.mass_ledger_state()
     header = {"format": FORMAT, "shape": list(plan.shape), "xmax": list(plan.xmax), "precision": plan.precision,
               "consistent_grid": bool(plan.consistent_grid), "hbar_over_m": plan.hbar_over_m,
               "four_pi_G": plan.four_pi_G, "self_gravity": bool(getattr(plan, "self_gravity", True)),
               "t": float(t), "step": int(step), "dt": None if dt is None else float(dt),
-              "average_count": int(avg_n), "meta": meta or {}}
+              "average_count": int(avg_n), "mass_ledger": ledger_on, "mass_reference": m_ref,
+              "meta": meta or {}}
     arrays = {k: np.asfortranarray(v) for k, v in arrays.items()}
     with open(path, "wb") as f:
         np.savez(f, header=np.frombuffer(json.dumps(header).encode(), dtype=np.uint8), **arrays)
@@ -63,23 +79,39 @@ def load(path):
     return header
 
 
+def _restore_state(plan, snap):
+    plan.set_psi(snap["psi"])
+    if snap.get("mass_ledger"):
+        plan.set_mass_ledger(True)
+        if snap.get("mass_reference") is not None:
+            plan.set_mass_reference(snap["mass_reference"])
+    elif snap.get("mass_ledger") is False:
+        plan.set_mass_ledger(False)
+    if "psi2_sum" in snap and snap.get("average_count", 0) > 0:
+        plan.density_restore(snap["psi2_sum"], snap["average_count"])
+    else:
+        plan.density_reset()
+
+
 def plan_from(snap, *, lib=None):
     """A new Plan with the snapshot's grid, precision and coefficients, holding
-    its psi (potential models and absorber are installed by the caller)."""
+    its psi, mass projection state and density average (potential models and
+    absorber are installed by the caller)."""
     from .plan import Plan
     p = Plan(tuple(snap["shape"]), tuple(snap["xmax"]), snap["precision"], consistent_grid=snap["consistent_grid"],
              hbar_over_m=snap["hbar_over_m"], four_pi_G=snap["four_pi_G"], self_gravity=snap["self_gravity"],
              lib=lib)
-    p.set_psi(snap["psi"])
+    _restore_state(p, snap)
     return p
 
 
 def restore(plan, snap):
-    """Load the snapshot's psi into an existing plan of the same grid; returns
-    (t, step).  A restored run continues with kick k at t + k*dt."""
+    """Load the snapshot's psi, mass projection state and density average into
+    an existing plan of the same grid; returns (t, step).  A restored run
+    continues with kick k at t + k*dt."""
     if tuple(snap["shape"]) != tuple(plan.shape) or snap["precision"] != plan.precision:
         raise ValueError("snapshot grid or precision differs from the plan")
-    plan.set_psi(snap["psi"])
+    _restore_state(plan, snap)
     return snap["t"], snap["step"]
 
 
@@ -91,17 +123,29 @@ VIRIAL_COLUMNS = ("PE_abs", "KE", "QE", "Virial", "MomentumX", "MomentumY", "Mom
 class PropTable:
     """Rows of (t, diagnostics) -> Prop.csv.  ``scale`` maps a column name to
     the unit factor the reference applies when it records it (uL for the radii,
-    uMomentum for the momenta; src/simulation_main.jl:557-558,642-650)."""
+    uMomentum for the momenta; src/simulation_main.jl:557-558,642-650).
 
-    def __init__(self, virial=True, scale=None):
+    TotalMass lags the other columns by one row, as in the reference: the loop
+    pushes ``total_halo_mass`` (src/simulation_main.jl:557) before it
+    recomputes it from the new state (:652), so row k holds the mass of the
+    state of row k-1 and the first row the initial condition's mass
+    (``initial_mass``, the reference's :349), with an infinite mass recorded
+    as 0.  Without ``initial_mass`` the first row takes its own state's mass."""
+
+    def __init__(self, virial=True, scale=None, initial_mass=None):
         self.columns = PROP_COLUMNS + (VIRIAL_COLUMNS if virial else ())
         self.scale = dict(scale or {})
         self.rows = []
+        self._mass_prev = None if initial_mass is None else float(initial_mass)
 
     def append(self, t, diag):
         row = {"t": float(t)}
         for c in self.columns[1:]:
-            row[c] = float(diag[c]) * self.scale.get(c, 1.0)
+            if c != "TotalMass":
+                row[c] = float(diag[c]) * self.scale.get(c, 1.0)
+        m = float(diag["TotalMass"]) if self._mass_prev is None else self._mass_prev
+        row["TotalMass"] = (0.0 if math.isinf(m) else m) * self.scale.get("TotalMass", 1.0)
+        self._mass_prev = float(diag["TotalMass"])
         self.rows.append(row)
 
     def write(self, path):
