@@ -75,8 +75,10 @@ def measured_traffic(shape, kernel):
         return json.load(f)["bytes_per_launch"].get("FP32 " + kernel)
 
 
-def energy(red):
-    return red["kinetic"] + red["w_self"] + red["v_ext"]
+def energy(W, red, dt):
+    """The energy the reference split (one kick of dt/2 per drift of dt)
+    conserves: K + (dt_kick/dt)(W_self + V_ext) (wavedm_amd.kdk.split_energy)."""
+    return W.kdk.split_energy(red, dt, 0.5 * dt)
 
 
 def run_gpu(W, cfg, precision, steps, warmup, ledger=False, diagnostics=True, round_ic=False):
@@ -296,6 +298,89 @@ def ensemble_leg(W, steps=None):
             p.close()
 
 
+def c1_leg(W, steps=None):
+    """The c1 workload (256^3, 64 solitons, static Hernquist host,
+    self-gravity, autoset dt, 500 steps): FP32 (with and without the mass
+    projection) against FP64 from the same complex64-rounded initial state,
+    and FP64 from the unrounded state (how much the dynamics amplify a
+    rounding-sized perturbation).  Reports psi relative L2, relative mass drift
+    and relative drift of the split's energy K + (dt_kick/dt)(W + V_ext),
+    which the static potential conserves up to the splitting error."""
+    cfg = W.configs.build("c1")
+    nsteps = steps or cfg["steps"]
+    res = {}
+    with W.Plan(cfg["shape"], cfg["xmax"], "f64", hbar_over_m=cfg["hbar_over_m"],
+                four_pi_G=cfg["four_pi_G"]) as p:
+        W.configs.install(p, cfg)
+        dt, _ = W.configs.autoset_dt(p, cfg["steps"])
+        psi_exact = p.get_psi()
+    psi0 = psi_exact.astype(np.complex64)
+    for label, prec, ledger, start in (("fp64", "f64", False, psi0), ("fp64_unrounded_ic", "f64", False, psi_exact),
+                                       ("fp32", "f32", False, psi0), ("fp32_mass_projection", "f32", True, psi0)):
+        with W.Plan(cfg["shape"], cfg["xmax"], prec, hbar_over_m=cfg["hbar_over_m"],
+                    four_pi_G=cfg["four_pi_G"]) as p:
+            W.configs.install(p, cfg)
+            p.set_psi(start)
+            p.set_mass_ledger(ledger)
+            r0 = p.reduce()
+            p.synchronize()
+            t0 = time.perf_counter()
+            p.step(nsteps, dt)
+            p.synchronize()
+            wall = time.perf_counter() - t0
+            r1 = p.reduce()
+            e0, e1 = energy(W, r0, dt), energy(W, r1, dt)
+            res[label] = {"ms_per_step": 1e3 * wall / nsteps, "mass_drift": abs(r1["mass"] - r0["mass"]) / r0["mass"],
+                          "energy_drift": abs(e1 - e0) / abs(e0), "energy0": e0, "energy1": e1,
+                          "psi": p.get_psi()}
+    ref = res["fp64"].pop("psi")
+    for k in ("fp64_unrounded_ic", "fp32", "fp32_mass_projection"):
+        res[k]["psi_rel_l2_vs_fp64"] = float(np.linalg.norm(res[k].pop("psi") - ref) / np.linalg.norm(ref))
+    res["workload"] = (f"c1: 256^3, L=40 kpc, 64 solitons + static Hernquist host, self-gravity, {nsteps} "
+                       f"steps, dt={dt:.6g} Gyr (autoset)")
+    res["speedup_fp32_vs_fp64"] = res["fp64"]["ms_per_step"] / res["fp32"]["ms_per_step"]
+    res["speedup_fp32_projection_vs_fp64"] = res["fp64"]["ms_per_step"] / res["fp32_mass_projection"]["ms_per_step"]
+    return res
+
+
+def error_curve(W, name="c2", checkpoints=(10, 20, 50, 100, 200, 500, 1000)):
+    """psi relative L2 against FP64 along a run: FP32 (with the mass
+    projection and without) and FP64 started from the complex64-rounded state
+    (the dynamics' own amplification of a rounding-sized perturbation), all
+    against FP64 from the exact device initial state, at each checkpoint."""
+    cfg = W.configs.build(name)
+    mk = lambda prec: W.Plan(cfg["shape"], cfg["xmax"], prec, hbar_over_m=cfg["hbar_over_m"],  # noqa: E731
+                             four_pi_G=cfg["four_pi_G"])
+    plans = {"fp64": mk("f64"), "fp64_rounded_ic": mk("f64"), "fp32": mk("f32"), "fp32_mass_projection": mk("f32")}
+    try:
+        for p in plans.values():
+            W.configs.install(p, cfg)
+        dt, _ = W.configs.autoset_dt(plans["fp64"], cfg["steps"])
+        psi0 = plans["fp64"].get_psi().astype(np.complex64)
+        for k in ("fp64_rounded_ic", "fp32", "fp32_mass_projection"):
+            plans[k].set_psi(psi0)
+        plans["fp32_mass_projection"].set_mass_ledger(True)
+        rows, done = [], 0
+        m0 = {k: p.reduce(1)["mass"] for k, p in plans.items()}
+        for c in checkpoints:
+            for p in plans.values():
+                p.step(c - done, dt, t0=done * dt)
+            done = c
+            ref = plans["fp64"].get_psi()
+            nref = np.linalg.norm(ref)
+            row = {"step": c}
+            for k in ("fp64_rounded_ic", "fp32", "fp32_mass_projection"):
+                row[k] = float(np.linalg.norm(plans[k].get_psi() - ref) / nref)
+                row[k + "_mass_drift"] = abs(plans[k].reduce(1)["mass"] - m0[k]) / m0[k]
+            del ref
+            rows.append(row)
+            print(json.dumps(row), file=sys.stderr, flush=True)
+        return {"workload": f"{name} {cfg['shape']}, dt={dt:.6g} Gyr", "rows": rows}
+    finally:
+        for p in plans.values():
+            p.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -307,6 +392,10 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-adjoint", action="store_true")
     ap.add_argument("--skip-ensemble", action="store_true")
+    ap.add_argument("--skip-c1", action="store_true")
+    ap.add_argument("--error-curve", action="store_true",
+                    help="only run the psi-error-versus-step curve of the workload (FP32 and rounded-IC FP64 "
+                         "against FP64) and print it as JSON")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -318,6 +407,9 @@ def main():
         copy_calibration(args.copy_calibration, max(args.steps, 1))
         return
     W = load_pkg()
+    if args.error_curve:
+        print(json.dumps(error_curve(W, args.config)), flush=True)
+        return
     cfg = W.configs.build(args.config)
     n = int(np.prod(cfg["shape"]))
     print(f"[bench] {cfg['name']} {cfg['shape']} FP32 warmup={args.warmup} steps={args.steps}", file=sys.stderr,
@@ -361,8 +453,10 @@ def main():
                        "preceding 100-step run with events on every kernel",
     }
     m0, m1 = r32["red0"]["mass"], r32["red1"]["mass"]
-    out["fp32"] = {"mass_drift": abs(m1 - m0) / m0, "energy0": energy(r32["red0"]),
-                   "energy1": energy(r32["red1"])}
+    out["fp32"] = {"mass_drift": abs(m1 - m0) / m0, "energy0": energy(W, r32["red0"], r32["dt"]),
+                   "energy1": energy(W, r32["red1"], r32["dt"]),
+                   "energy_note": "K + (dt_kick/dt)(W_self + V_ext), the split's energy; the c2 tidal field "
+                                  "rotates, so it is not conserved here (see the c1 leg)"}
     print("[bench] FP32 leg with the mass projection", file=sys.stderr, flush=True)
     rl = run_gpu(W, cfg, "f32", args.steps, min(args.warmup, 5), ledger=True, diagnostics=False)
     out["fp32"]["mass_projection"] = {
@@ -377,9 +471,10 @@ def main():
         r64 = run_gpu(W, cfg, "f64", args.steps, min(args.warmup, 5))
         ms64 = 1e3 * r64["wall"] / args.steps
         err = float(np.linalg.norm(r32["psi"] - r64["psi"]) / np.linalg.norm(r64["psi"]))
-        e32, e64 = energy(r32["red1"]), energy(r64["red1"])
+        e32, e64 = energy(W, r32["red1"], r32["dt"]), energy(W, r64["red1"], r64["dt"])
         out["fp64"] = {"ms_per_step": ms64, "value": n * args.steps / r64["wall"],
                        "speedup_fp32_vs_fp64": ms64 / ms32,
+                       "speedup_fp32_projection_vs_fp64": ms64 / out["fp32"]["mass_projection"]["ms_per_step"],
                        "psi_rel_l2_fp32_vs_fp64": err,
                        "mass_drift_fp64": abs(r64["red1"]["mass"] - r64["red0"]["mass"]) / r64["red0"]["mass"],
                        "energy_rel_diff_fp32_vs_fp64": abs(e32 - e64) / abs(e64),
@@ -392,6 +487,9 @@ def main():
         del rr
         del r64
     del r32
+    if not args.skip_c1:
+        print("[bench] c1 leg (256^3, 500 steps, FP32 vs FP64)", file=sys.stderr, flush=True)
+        out["c1"] = c1_leg(W)
     if not args.skip_adjoint:
         print("[bench] adjoint leg (c4)", file=sys.stderr, flush=True)
         out["adjoint"] = adjoint_leg(W)

@@ -81,6 +81,7 @@ extern "C" {
                               V = 1/2 (x-c)^T R(t) T R(t)^T (x-c), R(t) rotation about n
                               by 2 pi t / period */
 #define WDM_V_KAPPA 8      /* params: kappa                      V = kappa |psi|^2 */
+#define WDM_V_TIDAL 16     /* set by wdm_set_tidal_model (not wdm_set_potential_model) */
 
 /* wdm_reduce output slots */
 #define WDM_RED_MASS 0    /* sum |psi|^2 dV                                    */
@@ -140,6 +141,22 @@ int wdm_set_border(wdm_plan* plan, const double* border, int rank);
 int wdm_set_static_potential(wdm_plan* plan, const double* V);
 int wdm_set_potential_model(wdm_plan* plan, int kind, const double* params, int nparams);
 int wdm_clear_potential_models(wdm_plan* plan);
+/* Milky Way tidal potential added to V in every kick (opt-in; replaces the
+   MW_tidal_interpolate branch of compute_tidal_potential,
+   src/physics/tidal_fields.jl:64-146, whose result the reference drops, :117).
+   table: tnx*tny*tnz FP64 potential (x fastest) on the uniform grid
+   lo[a] + i (hi[a] - lo[a]) / (tn_a - 1); traj_t[ntraj] non-decreasing
+   kick times and traj_xyz[3*ntraj] the offsets (df_traj rows over uL): the
+   kick at time t uses the last row with traj_t <= t (the first before the
+   table starts).  Per kick: V_tidal(x) = interp(x + o) minus its value at the
+   density peak, minus the ramps of its central differences there, plus the
+   rho-weighted mean of those ramps (cancel_field_gradient_at_center!,
+   :126-146).  Coordinates outside the table are clamped to it.  The table and
+   trajectory are copied; table == NULL clears; wdm_clear_potential_models
+   clears it too.  Not differentiated (wdm_adjoint_forward refuses it) and
+   time-dependent (ensemble graph mode refuses it). */
+int wdm_set_tidal_model(wdm_plan* plan, const double* table, int tnx, int tny, int tnz, const double* lo,
+                        const double* hi, const double* traj_t, const double* traj_xyz, int ntraj);
 
 /* phi_out = scale * lap^-1 (rho - <rho>) on the plan's periodic lattice
    (k = 2 pi m / (2 xmax), k = 0 -> 0).  rho == NULL: rho = |psi|^2 of the
@@ -284,6 +301,17 @@ int wdm_density_average(wdm_plan* plan, double* host_out, long long* count);
 int wdm_density_reset(wdm_plan* plan);
 int wdm_density_sum(wdm_plan* plan, double* host_sum, long long* count);
 int wdm_density_restore(wdm_plan* plan, const double* host_sum, long long count);
+
+/* The acceleration field of an initial condition (src/ic/generation.jl:246-291,
+   periodic gravity): Phi = 4 pi G lap^-1 (rho_halo - <rho_halo>) on the
+   plan's lattice (compute_acceleration_field), a = -grad Phi by central
+   differences (periodic on the faces), and a_CDM = |a| on the interior cells,
+   0 on the faces and at the 1-based cell (div(N,2), div(N,2), div(N,2))
+   (generate_velocity_field).  rho_halo and every output are host arrays of
+   nx*ny*nz doubles, x fastest; outputs may be NULL.  The plan's psi is kept;
+   its Poisson buffer is overwritten (the next kick recomputes Phi). */
+int wdm_ic_acceleration(wdm_plan* plan, const double* rho_halo, double* phi, double* ax, double* ay, double* az,
+                        double* a_cdm);
 
 /* Batched ensemble of independent plans (uneven grids; no counterpart in the
    reference, which steps one box at a time).  The plans stay owned by the

@@ -8,6 +8,20 @@ then the same along y (vy) and z (vz).  src/ic/generation.jl:330-336 removes
 the mass-weighted mean velocity first; src/ic/generation.jl:185-187 assembles
 psi = sqrt(rho) exp(i theta).
 
+The acceleration -> velocity step before it (src/ic/generation.jl:246-337):
+compute_acceleration_field (periodic branch :264-270): Phi = 4 pi *
+parallel_poisson(Delta, [N-1...], rho_halo) and a = grad_central(-Delta, Phi);
+generate_velocity_field's a_CDM (:285-291): |a| on the interior cells
+[2:end-1]^3, 0 on the faces, then a_CDM[div(end,2), div(end,2), div(end,2)]
+*= 0 (1-based); adjust_velocity_field's bulk_center_size zeroing (:316-326)
+on the 1-based ranges div(N,2) - div(s,2) + 1 : div(N,2) - div(s,2) + s.  The
+velocity formulas of that step (AstroIC.freefall_velocity_acc,
+rotational_velocity_acc) and the bulk_perturb filters (zoom, shrink) come
+from packages absent from the image: parity unpinned, taken as host inputs.
+grad_central comes from PhysicalFDM (absent): central differences with
+periodic neighbours on the faces here; a_CDM uses interior cells only, so it
+does not depend on that face rule.
+
 cumul_integrate comes from PhysicalParticles (absent from the image); it is
 restated as the cumulative trapezoid rule of NumericalIntegration.jl's
 published algorithm, out[0] = 0, out[i] = 1/2 sum_{l<=i} (x[l] - x[l-1])
@@ -61,3 +75,39 @@ def ic_from_velocity(rho, vx, vy, vz, d, hbar_over_m=1.0, remove_bulk=True):
         vx, vy, vz = remove_bulk_velocity(rho, vx, vy, vz)
     theta = solve_vector_equation(vx, vy, vz, *d)
     return np.sqrt(rho) * np.exp(1j * theta / hbar_over_m)
+
+
+def compute_acceleration_field(rho_halo, delta, four_pi_G=4.0 * np.pi):
+    """(Phi, ax, ay, az) of src/ic/generation.jl:264-272 (periodic)."""
+    from oracle import ssfm_oracle as O
+    from oracle import diag_oracle as D
+    rho = np.asarray(rho_halo, dtype=np.float64)
+    nx, ny, nz = rho.shape
+    poisson_N = (max(nx - 1, 1), max(ny - 1, 1), max(nz - 1, 1))
+    phi = four_pi_G * O.parallel_poisson(delta, poisson_N, rho - rho.mean()).real
+    acc = tuple(-D.central_diff(phi, a, delta[a]) for a in range(3))
+    return (phi,) + acc
+
+
+def a_cdm(ax, ay, az):
+    """generate_velocity_field's acceleration magnitude (src/ic/generation.jl:289-291)."""
+    out = np.zeros(np.shape(ax))
+    sl = (slice(1, -1),) * 3
+    out[sl] = np.sqrt(ax[sl] ** 2 + ay[sl] ** 2 + az[sl] ** 2)
+    c = tuple(n // 2 - 1 for n in out.shape)  # Julia div(end, 2), 1-based
+    out[c] *= 0
+    return out
+
+
+def zero_center_velocity(vx, vy, vz, bulk_center_size):
+    """adjust_velocity_field's bulk_center_size zeroing (src/ic/generation.jl:316-326)."""
+    out = [np.array(v, dtype=np.float64, copy=True) for v in (vx, vy, vz)]
+    if bulk_center_size > 0:
+        rng = []
+        for n in out[0].shape:
+            lo = n // 2 - bulk_center_size // 2 + 1          # 1-based first
+            hi = n // 2 - bulk_center_size // 2 + bulk_center_size  # 1-based last
+            rng.append(slice(lo - 1, hi))
+        for v in out:
+            v[tuple(rng)] = 0.0
+    return tuple(out)

@@ -19,7 +19,8 @@ SUBSET = ("roundtrip or (poisson and (8x8x8 or 32x16x8 or 160x16x8 or 12x8x384))
           "device_matches_golden or (diagnostics_match_oracle and (32x24x16 or 16x12x8 or 96x48x12)) or "
           "diagnostics_after or diagnostics_error or restart_continues or restart_with_mass or mass_reference or "
           "density_average or "
-          "projection_error_paths or projection_reference_follows or projection_enabled or ic_velocity or baryon_mesh")
+          "projection_error_paths or projection_reference_follows or projection_enabled or ic_velocity or "
+          "ic_acceleration or velocity_ic_chain or baryon_mesh or (tidal_kicks and 32x24x16) or tidal_refusals")
 
 
 def test_gpu_suite_on_emulated_kernels(wdm):
@@ -39,6 +40,7 @@ def test_gpu_suite_on_emulated_kernels(wdm):
                         os.path.join(ROOT, "tests", "test_gpu_snapshot.py"),
                         os.path.join(ROOT, "tests", "test_gpu_ledger.py"),
                         os.path.join(ROOT, "tests", "test_gpu_ic.py"),
+                        os.path.join(ROOT, "tests", "test_gpu_tidal.py"),
                         "-m", "gpu", "--emu", "-q", "-x", "-p", "no:cacheprovider", "-k", SUBSET],
                        cwd=ROOT, env=env, capture_output=True, text=True, timeout=900)
     tail = "\n".join(l for l in (r.stdout + r.stderr).splitlines() if "WARNING: ASan" not in l)[-3000:]

@@ -151,7 +151,7 @@ def test_adjoint_tape_refuses_changed_plan(wdm, nativelib):
         p.adjoint_forward(4, pr.dt, ckpt_every=2)
         g0a, gpa = p.adjoint_backward(g)
         g0b, gpb = p.adjoint_backward(g)   # same tape, unchanged plan
-        assert np.array_equal(g0a, g0b) and np.array_equal(gpa, gpb)
+        assert np.array_equal(g0a, g0b) and np.array_equal(gpa, gpb, equal_nan=True)
         for change in ("hernquist", "coeffs", "set_psi", "step"):
             p.set_psi(psi0)
             p.adjoint_forward(4, pr.dt, ckpt_every=2)
@@ -175,4 +175,4 @@ def test_adjoint_tape_refuses_changed_plan(wdm, nativelib):
         p.set_psi(psi0)
         p.adjoint_forward(4, pr.dt, ckpt_every=2)
         g0c, gpc = p.adjoint_backward(g)
-        assert np.array_equal(g0a, g0c) and np.array_equal(gpa, gpc)
+        assert np.array_equal(g0a, g0c) and np.array_equal(gpa, gpc, equal_nan=True)

@@ -47,3 +47,52 @@ def test_ic_velocity_rejects_zero_density(wdm, nativelib):
         assert e.value.code == WDM_ERR_ARG
         p.set_ic_velocity(z, z, z, z, remove_bulk=False)  # no mean to take: psi = 0
         assert np.all(p.get_psi() == 0)
+
+
+@pytest.mark.parametrize("shape,prec", [((32, 24, 16), "f64"), ((48, 32, 24), "f32"), ((24, 16, 12), "f64")])
+def test_ic_acceleration_matches_oracle(wdm, nativelib, shape, prec):
+    """wdm_ic_acceleration (compute_acceleration_field + a_CDM,
+    src/ic/generation.jl:246-291) against the oracle: Phi, a and a_CDM.
+    Bars: FP64 <= 1e-12 relative to each field's maximum, FP32 <= 2e-6."""
+    X = (3.0, 2.5, 2.0)
+    rho, _, d = _fields(shape, X, 7)
+    ref = dict(zip(("phi", "ax", "ay", "az"), I.compute_acceleration_field(rho, d, 4 * np.pi * 0.6)))
+    ref["a_cdm"] = I.a_cdm(ref["ax"], ref["ay"], ref["az"])
+    with wdm.Plan(shape, X, prec, four_pi_G=4 * np.pi * 0.6, lib=nativelib) as p:
+        p.set_psi(np.sqrt(rho) + 0j)
+        got = wdm.ic.compute_acceleration_field(p, rho)
+        # psi is kept, and the next kick's Phi is that of psi again
+        assert np.allclose(p.get_psi(), np.sqrt(rho), rtol=0, atol=1e-6 * np.sqrt(rho).max())
+    tol = 1e-12 if prec == "f64" else 2e-6
+    for k, v in ref.items():
+        err = np.abs(got[k] - v).max() / np.abs(v).max()
+        print(f"{prec} {shape} {k}: {err:.3e}", flush=True)
+        assert err <= tol, (k, err)
+    assert got["a_cdm"][shape[0] // 2 - 1, shape[1] // 2 - 1, shape[2] // 2 - 1] == 0.0
+    assert not got["a_cdm"][0].any() and not got["a_cdm"][:, :, -1].any()
+
+
+def test_velocity_ic_chain_matches_oracle(wdm, nativelib):
+    """The whole chain with a caller-supplied velocity formula (here a
+    rotation v = omega x r scaled by a_CDM): device accelerations, centre
+    zeroing (bulk_center_size 4), bulk removal, line integrals, psi; against
+    the oracle chain.  FP64 <= 1e-12."""
+    shape, X = (32, 24, 16), (3.0, 2.5, 2.0)
+    rho, _, d = _fields(shape, X, 9)
+
+    def velocity(Xg, Yg, Zg, a):
+        return -0.3 * Yg * a, 0.3 * Xg * a, 0.05 * a
+
+    x, y, z, _, _ = O.setup_grid(*X, *shape)
+    Xg, Yg, Zg = np.meshgrid(x, y, z, indexing="ij")
+    _, ax, ay, az = I.compute_acceleration_field(rho, d, 4 * np.pi)
+    v = I.zero_center_velocity(*velocity(Xg, Yg, Zg, I.a_cdm(ax, ay, az)), 4)
+    ref = I.ic_from_velocity(rho, *v, d)
+    with wdm.Plan(shape, X, "f64", lib=nativelib) as p:
+        wdm.ic.velocity_ic(p, rho, velocity, bulk_center_size=4)
+        got = p.get_psi()
+        with pytest.raises(NotImplementedError):
+            wdm.ic.velocity_ic(p, rho, velocity, bulk_perturb=True)
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print(f"velocity IC chain: {err:.3e}", flush=True)
+    assert err <= 1e-12

@@ -268,6 +268,38 @@ def test_reductions_and_cfl_match_oracle(wdm, nativelib):
         assert math.isclose(a, b, rel_tol=1e-12)
 
 
+@pytest.mark.parametrize("shape,prec", [((24, 16, 12), "f64"), ((48, 32, 1), "f64"), ((32, 24, 16), "f32")])
+def test_reductions_with_external_potential(wdm, nativelib, shape, prec):
+    """wdm_reduce's read-only passes against the oracle's sums: mass, the
+    spectral kinetic energy (per-axis Parseval), W = 1/2 sum rho Phi and
+    V_ext = sum rho V with a Hernquist host and a static field, max |psi| and
+    max |Phi|; an axis of length 1 contributes no kinetic term."""
+    X = (3.0, 2.5, 2.0)
+    psi = blob(shape, X, seed=12)
+    x, y, z, delta, dV = O.setup_grid(*X, *shape)
+    xxx, yyy, zzz, _ = O.setup_coordinates(x, y, z)
+    G4 = 4 * np.pi * 0.7
+    hq = (9.0, 0.8, (0.3, -0.2, 0.1))
+    Vh = -(G4 / (4 * np.pi)) * hq[0] / (np.sqrt((xxx - hq[2][0]) ** 2 + (yyy - hq[2][1]) ** 2
+                                               + (zzz - hq[2][2]) ** 2) + hq[1])
+    Vs = 0.05 * np.cos(xxx) * yyy
+    with wdm.Plan(shape, X, prec, four_pi_G=G4, lib=nativelib) as p:
+        p.set_psi(psi)
+        p.set_hernquist(hq[0], hq[1], hq[2])
+        p.set_static_potential(Vs)
+        red = p.reduce()
+        got = p.get_psi()  # the plan's own (possibly rounded) state
+    K, W, Vx = O.energies(got, X, four_pi_G=G4, V_field=Vh + Vs)
+    phi = O.compute_gravitational_potential(got, delta, G4).real
+    tol = 1e-12 if prec == "f64" else 2e-6
+    assert math.isclose(red["mass"], O.mass(got, dV), rel_tol=tol)
+    assert math.isclose(red["kinetic"], K, rel_tol=tol)
+    assert math.isclose(red["w_self"], W, rel_tol=tol)
+    assert math.isclose(red["v_ext"], Vx, rel_tol=tol)
+    assert math.isclose(red["psi_max"], float(np.abs(got).max()), rel_tol=tol)
+    assert math.isclose(red["phi_max"], float(np.abs(phi).max()), rel_tol=max(tol, 1e-6 if prec == "f32" else 0))
+
+
 def test_fp32_against_fp64_oracle(wdm, nativelib):
     """FP32 plan vs FP64 oracle, 64^3, 50 steps of the C0 recipe."""
     U, IC = wdm.units, wdm.ic

@@ -34,6 +34,7 @@ WDM_SPLIT_STRANG = 1
 WDM_V_HERNQUIST = 2
 WDM_V_QUADRUPOLE = 4
 WDM_V_KAPPA = 8
+WDM_V_TIDAL = 16
 WDM_RED_MASS = 0
 WDM_RED_KINETIC = 1
 WDM_RED_WSELF = 2
@@ -77,6 +78,7 @@ SIGNATURES = {
     "wdm_set_static_potential": (_I, [_P, _DP]),
     "wdm_set_potential_model": (_I, [_P, _I, _DP, _I]),
     "wdm_clear_potential_models": (_I, [_P]),
+    "wdm_set_tidal_model": (_I, [_P, _DP, _I, _I, _I, _DP, _DP, _DP, _DP, _I]),
     "wdm_poisson": (_I, [_P, _DP, _DP, _D]),
     "wdm_get_phi": (_I, [_P, _DP]),
     "wdm_kick": (_I, [_P, _D, _D]),
@@ -93,6 +95,7 @@ SIGNATURES = {
     "wdm_set_mass_ledger": (_I, [_P, _I]),
     "wdm_get_mass_ledger": (_I, [_P, C.POINTER(_I), _DP]),
     "wdm_set_mass_reference": (_I, [_P, _D]),
+    "wdm_ic_acceleration": (_I, [_P, _DP, _DP, _DP, _DP, _DP, _DP]),
     "wdm_ic_velocity": (_I, [_P, _DP, _DP, _DP, _DP, _I]),
     "wdm_step_timed_mask": (_I, [_P, _I, C.c_double, C.c_double, C.c_double, C.c_uint, _DP, _I, _DP]),
     "wdm_cfl": (_I, [_P, _D, _DP]),

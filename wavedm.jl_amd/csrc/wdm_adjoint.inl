@@ -181,6 +181,9 @@ template <class T>
 void adj_forward(wdm_plan* p, int nsteps, double dt, double dt_kick, double t0, int K) {
     if (p->kspace) fail(WDM_ERR_STATE, "wdm_adjoint_forward: psi is in k-space");
     if (p->ledger) fail(WDM_ERR_UNSUPPORTED, "wdm_adjoint_forward: the mass projection is not differentiated");
+    if (p->vflags & V_TIDAL)
+        fail(WDM_ERR_UNSUPPORTED, "wdm_adjoint_forward: the tidal potential (centred on the density peak) is not "
+                                  "differentiated");
     AdjState& A = adj_of(p);
     A.ready = false;
     const size_t n = p->ncell;

@@ -126,7 +126,9 @@ std::vector<unsigned> ens_gens(const wdm_ensemble* e) {
     return g;
 }
 
-bool time_dependent_v(const wdm_plan* p) { return (p->vflags & V_QUADRUPOLE) && p->quad[9] != 0.0; }
+bool time_dependent_v(const wdm_plan* p) {
+    return ((p->vflags & V_QUADRUPOLE) && p->quad[9] != 0.0) || (p->vflags & V_TIDAL);
+}
 
 void ens_prepare(wdm_ensemble* e, const double* dt) {
     for (size_t i = 0; i < e->m.size(); ++i) {
@@ -167,6 +169,12 @@ void ens_step(wdm_ensemble* e, int nsteps, const double* dt, const double* dtk, 
                           e->g_dtk == std::vector<double>(dtk, dtk + nm) && e->g_gen == ens_gens(e);
         if (!same) {
             ens_drop_graph(e);
+            // a member with the mass projection ends every step with the final
+            // scaling of psi, after which the next step recomputes Phi from
+            // the returned psi (as wdm_step does per call): the captured step
+            // must start with that Poisson solve
+            for (wdm_plan* p : e->m)
+                if (p->ledger) p->phi_valid = false;
             hip_check(hipStreamBeginCapture(e->root, hipStreamCaptureModeRelaxed), "begin capture");
             try {
                 ens_issue(e, dt, dtk, t0, 0);

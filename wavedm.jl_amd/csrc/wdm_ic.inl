@@ -124,9 +124,71 @@ void ic_velocity(wdm_plan* p, const double* rho, const double* const v[3], int r
     p->ledger_stale = true;
 }
 
+// The acceleration field of an initial condition (src/ic/generation.jl:246-273,
+// periodic branch): Phi = 4 pi G lap^-1 (rho_halo - <rho_halo>) on the plan's
+// lattice, a = -grad_central(Phi) (central differences, periodic on the
+// faces, as wdm_acceleration); and generate_velocity_field's magnitude
+// (:285-291): a_CDM = |a| on the interior cells [2:end-1]^3, 0 on the faces,
+// times 0 at the cell (div(N,2), div(N,2), div(N,2)) (1-based).  The interior
+// uses no face neighbour, so a_CDM does not depend on the face rule.
+WDM_KERNEL(256)
+void k_ic_acc_cdm(const double* __restrict__ phi, double* __restrict__ out, int nx, int ny, int nz, double hx,
+                  double hy, double hz) {
+    WDM_FOR_CELLS(nx, ny, nz) {
+        double a = 0.0;
+        if (x > 0 && x < nx - 1 && y > 0 && y < ny - 1 && z > 0 && z < nz - 1) {
+            const unsigned sy = unsigned(nx), sz = unsigned(nx) * unsigned(ny);
+            const double ax = -(phi[i + 1] - phi[i - 1]) * hx;
+            const double ay = -(phi[i + sy] - phi[i - sy]) * hy;
+            const double az = -(phi[i + sz] - phi[i - sz]) * hz;
+            a = __builtin_sqrt(ax * ax + ay * ay + az * az);
+        }
+        if (x == nx / 2 - 1 && y == ny / 2 - 1 && z == nz / 2 - 1) a *= 0.0;
+        out[i] = a;
+    }
+}
+
+template <class T>
+void ic_acceleration(wdm_plan* p, const double* rho, double* phi_out, double* const acc_out[3], double* a_cdm) {
+    poisson_api<T>(p, rho, nullptr, p->four_pi_G);  // rhohat = Phi_hat of rho_halo
+    typedef typename TwEntry<T>::type TwE;
+    double* phi = p->scratch_real.get<double>(p->ncell);
+    ops_of<T>(p, 0)->x_c2r_real(p->rhohat.as<cx<T>>(), phi, p->ny * p->nz, p->hpitch, p->ax[0].tw.as<TwE>(),
+                                 p->stream);
+    hip_check(hipGetLastError(), "C2R pass");
+    if (phi_out)
+        hip_check(hipMemcpyAsync(phi_out, phi, sizeof(double) * p->ncell, hipMemcpyDeviceToHost, p->stream), "phi copy");
+    double* g = p->diag.get<double>(p->ncell);
+    for (int a = 0; a < 3; ++a) {
+        if (!acc_out[a]) continue;
+        hip_check(hipStreamSynchronize(p->stream), "acceleration sync");  // g is reused per axis
+        WDM_LAUNCH((k_neg_cdiff), dim3(adj_line_blocks(p->ny, p->nz)), dim3(256), p->stream, (const double*)phi, g,
+                   p->nx, p->ny, p->nz, a, 0.5 / p->dx[a]);
+        hip_check(hipGetLastError(), "acceleration");
+        hip_check(hipMemcpyAsync(acc_out[a], g, sizeof(double) * p->ncell, hipMemcpyDeviceToHost, p->stream),
+                  "acceleration copy");
+    }
+    if (a_cdm) {
+        hip_check(hipStreamSynchronize(p->stream), "a_CDM sync");
+        WDM_LAUNCH((k_ic_acc_cdm), dim3(adj_line_blocks(p->ny, p->nz)), dim3(256), p->stream, (const double*)phi, g,
+                   p->nx, p->ny, p->nz, 0.5 / p->dx[0], 0.5 / p->dx[1], 0.5 / p->dx[2]);
+        hip_check(hipGetLastError(), "a_CDM");
+        hip_check(hipMemcpyAsync(a_cdm, g, sizeof(double) * p->ncell, hipMemcpyDeviceToHost, p->stream), "a_CDM copy");
+    }
+    hip_check(hipStreamSynchronize(p->stream), "ic acceleration sync");
+}
+
 }  // namespace
 
 extern "C" {
+
+int wdm_ic_acceleration(wdm_plan* p, const double* rho_halo, double* phi, double* ax, double* ay, double* az,
+                        double* a_cdm) {
+    if (!p || !rho_halo) return fail_code(p, WDM_ERR_ARG, "wdm_ic_acceleration: NULL argument");
+    if (p->kspace) return fail_code(p, WDM_ERR_STATE, "wdm_ic_acceleration: psi is in k-space");
+    double* const acc[3] = {ax, ay, az};
+    return guarded(p, [&] { DISPATCH(p, ic_acceleration, p, rho_halo, phi, acc, a_cdm); });
+}
 
 int wdm_ic_velocity(wdm_plan* p, const double* rho, const double* vx, const double* vy, const double* vz,
                     int remove_bulk) {

@@ -13,6 +13,7 @@
 #pragma once
 
 #include "wdm_lines.hpp"
+#include "wdm_tidal.hpp"
 
 namespace wdm {
 
@@ -34,6 +35,14 @@ typedef Comp PoissonPolicy;
 typedef Plain PoissonPolicy;
 #endif
 
+// Arithmetic policy of the psi passes in FP32 plans: compensated twiddles
+// (hi/lo pairs) by default; WDM_PSI_PLAIN builds single FP32 twiddles.
+#ifdef WDM_PSI_PLAIN
+typedef Plain PsiPolicyF32;
+#else
+typedef Comp PsiPolicyF32;
+#endif
+
 // COL_FMIS: COL_FMI that also stores the forward spectrum to KTabs::spec_out
 enum ColMode { COL_FWD = 0, COL_INV = 1, COL_FMI = 2, COL_MI = 3, COL_FMIS = 4 };
 // KF_PNONE: no factor, Plain arithmetic (the y passes of the Poisson chain)
@@ -43,10 +52,11 @@ enum KFactor { KF_NONE = 0, KF_DRIFT = 1, KF_POISSON = 2, KF_PNONE = 3 };
 //   KF_DRIFT:   complex FP64 exp(-i (hbar/m) k_a^2 dt / 2) per axis,
 //               multiplied together and by `scale` (= 1/N).  The column x other
 //               product is formed once per lane; each element's full factor is
-//               formed in FP64 and, in FP32 plans, split into hi/lo for one
-//               compensated multiply.  (Two compensated FP32 multiplies, by the
-//               line factor and then by the lane factor, bias |psi|^2 by
-//               ~1e-10 per step where cos(k^2 dt) rounds to 1.0f.)
+//               formed in FP64 and, in FP32 plans, applied to the widened data
+//               in FP64 with one rounding per component (mul_dyn).  (Two
+//               compensated FP32 multiplies, by the line factor and then by the
+//               lane factor, bias |psi|^2 by ~1e-10 per step where cos(k^2 dt)
+//               rounds to 1.0f.)
 //   KF_POISSON: k_a^2 per axis (FP64, or FP32 in FP32 plans); factor =
 //               scale / (kc^2+ko^2+kl^2), 0 at k = 0 (scale = -4 pi G / N).
 struct KTabs {
@@ -129,7 +139,7 @@ WDM_FI cx<T> apply_kf(cx<T> v, const KfLane<KF, T>& l, const KTabs& kt, int k) {
 // ------------------------------------------------------------------ potentials
 // External potential models evaluated in-kernel (replacing the host V(x,y,z,psi)
 // callback of src/solver/KDK.jl:64).  All parameters FP64.
-enum VFlags { V_STATIC = 1, V_HERNQUIST = 2, V_QUADRUPOLE = 4, V_KAPPA = 8 };
+enum VFlags { V_STATIC = 1, V_HERNQUIST = 2, V_QUADRUPOLE = 4, V_KAPPA = 8, V_TIDAL = 16 };
 struct VModel {
     int flags;
     const double* vstatic;   // full grid, x-fastest (V_STATIC)
@@ -138,7 +148,15 @@ struct VModel {
     double qT[6];            // Txx Tyy Tzz Txy Txz Tyz at the kick time; V = 1/2 (x-c)^T T (x-c)
     double q_c[3];
     double kappa;            // V = kappa |psi|^2 (GNLSE term)
+    int gnx, gny;            // grid x and y lengths (cell indices of the tidal ramps)
+    TidalModel tidal;        // MW tidal potential of the kick (V_TIDAL)
 };
+
+WDM_HDFI double tidal_at(const VModel& vm, double x, double y, double z, unsigned idx) {
+    const int i = int(idx % unsigned(vm.gnx)), j = int((idx / unsigned(vm.gnx)) % unsigned(vm.gny));
+    const int k = int(idx / (unsigned(vm.gnx) * unsigned(vm.gny)));
+    return tidal_V(vm.tidal, x, y, z, i, j, k);
+}
 
 template <class T>
 WDM_FI double eval_V(const VModel& vm, double x, double y, double z, unsigned idx, cx<T> psi) {
@@ -154,6 +172,7 @@ WDM_FI double eval_V(const VModel& vm, double x, double y, double z, unsigned id
              vm.qT[3] * dx * dy + vm.qT[4] * dx * dz + vm.qT[5] * dy * dz;
     }
     if (vm.flags & V_KAPPA) V += vm.kappa * double(norm2(psi));
+    if (vm.flags & V_TIDAL) V += tidal_at(vm, x, y, z, idx);
     return V;
 }
 
@@ -164,9 +183,12 @@ WDM_FI double eval_V(const VModel& vm, double x, double y, double z, unsigned id
 struct LineV {
     double qa, qb, qc;  // quadrupole: (qa*dx + qb)*dx + qc
     double hr2;         // Hernquist: (y - c_y)^2 + (z - c_z)^2
+    double y, z;        // the line's coordinates (tidal term)
 };
 WDM_FI LineV line_V(const VModel& vm, double y, double z) {
     LineV l;
+    l.y = y;
+    l.z = z;
     const double dy = y - vm.q_c[1], dz = z - vm.q_c[2];
     l.qa = 0.5 * vm.qT[0];
     l.qb = vm.qT[3] * dy + vm.qT[4] * dz;
@@ -188,6 +210,7 @@ WDM_FI double eval_V_line(const VModel& vm, const LineV& l, double x, unsigned i
         V += fmaT(fmaT(l.qa, dx, l.qb), dx, l.qc);
     }
     if (vm.flags & V_KAPPA) V += vm.kappa * double(norm2(psi));
+    if (vm.flags & V_TIDAL) V += tidal_at(vm, x, l.y, l.z, idx);
     return V;
 }
 
@@ -988,64 +1011,173 @@ void k_x_fft(cx<T>* __restrict__ psi, int nlines, const typename TwEntry<T>::typ
     }
 }
 
-// ------------------------------------------------------------------ reductions
-// Sum of k^2 |psi_hat|^2 (kinetic energy numerator) and sum |psi_hat|^2 over a
-// fully transformed grid; per-block partial pairs.
-template <class T>
-WDM_KERNEL(256)
-void k_spectral_sums(const cx<T>* __restrict__ ph, int nx, int ny, int nz, const double* __restrict__ k2x,
-                     const double* __restrict__ k2y, const double* __restrict__ k2z, double* __restrict__ partials) {
-    WDM_SHARED double s1[256], s2[256];
-    const int t = int(threadIdx.x);
-    const size_t n = size_t(nx) * ny * nz;
-    double a = 0.0, b = 0.0;
-    for (size_t i = size_t(blockIdx.x) * 256 + t; i < n; i += size_t(gridDim.x) * 256) {
-        const int x = int(i % nx);
-        const int y = int((i / nx) % ny);
-        const int z = int(i / (size_t(nx) * ny));
-        const double p = double(norm2(ph[i]));
-        a += (k2x[x] + k2y[y] + k2z[z]) * p;
-        b += p;
+// ------------------------------------------------------------------ read-only reductions
+// The diagnostics of wdm_reduce / wdm_cfl as line passes that read the state
+// and write only per-wave partials: no N-sized scratch.  The spectral kinetic
+// sum splits by axis, sum_k k^2 |psi_hat_k|^2 = sum_a (N / n_a) sum k_a^2
+// |F_a psi|^2 over the grid (Parseval along the two other axes of the
+// unnormalised transform), so each pass needs only the 1-D transform of its
+// own lines.
+
+// Deterministic wave reduction (tree over the lanes in a fixed order) through
+// a 64-double LDS buffer of the wave; every lane returns the result.
+WDM_FI double wave_tree(double* buf, int lane, double v, bool is_max) {
+    buf[lane] = v;
+    WDM_WAVE_SYNC();
+    WDM_UNROLL
+    for (int s = kWave / 2; s > 0; s >>= 1) {
+        if (lane < s) {
+            const double a = buf[lane], b = buf[lane + s];
+            buf[lane] = is_max ? (b > a ? b : a) : a + b;
+        }
+        WDM_WAVE_SYNC();
     }
-    s1[t] = a;
-    s2[t] = b;
-    __syncthreads();
-    if (t == 0) {
-        double A = 0.0, B = 0.0;
-        for (int i = 0; i < 256; ++i) { A += s1[i]; B += s2[i]; }
-        partials[2 * blockIdx.x] = A;
-        partials[2 * blockIdx.x + 1] = B;
+    const double r = buf[0];
+    WDM_WAVE_SYNC();
+    return r;
+}
+
+// signed FFT-order frequency index of position k on an axis of length n
+WDM_HDFI int fft_signed(int k, int n) { return 2 * k < n ? k : k - n; }
+
+// partial slots of k_x_red, per launched wave
+enum RedSlot { RS_MASS = 0, RS_RHOMAX = 1, RS_RHOPHI = 2, RS_PHIMAX = 3, RS_RHOV = 4, RS_KX = 5, RS_COUNT = 6 };
+
+// x lines: rho = |psi|^2 (sum, max), Phi from the Poisson half spectrum by a
+// C2R in registers (sum rho Phi, max |Phi|; phihat null: no self-gravity),
+// V from the plan's models at the kick time in ka (sum rho V), and
+// sum kx^2 |F_x psi|^2 (kx in units of kscale^(1/2), FFT order).
+template <class T, class P, class F, int NW>
+WDM_KERNEL(NW * 64)
+void k_x_red(const cx<T>* __restrict__ psi, const cx<T>* __restrict__ phihat, int nlines, int hpitch,
+             const typename TwEntry<T>::type* __restrict__ tw, KickArgs ka, double kscale,
+             double* __restrict__ partials) {
+    constexpr int N1 = F::N1, N2 = F::N2, M = F::M, N = F::N;
+    WDM_SHARED cx<T> xbuf[NW][F::XBUF];
+    WDM_SHARED typename TwEntry<T>::type tw_s[F::N];
+    WDM_SHARED double rbuf[NW][kWave];
+    tw = stage_table<F::N>(tw_s, tw);
+    const LaneMap m = lane_map<F, NW>();
+    const int l = m.w * F::LPW + m.lw;
+    const bool ok = l < nlines;
+    const int lc = ok ? l : 0;
+    const int y = lc % ka.Ny, z = lc / ka.Ny;
+    const unsigned rbase = unsigned(lc) * unsigned(N);
+    cx<T>* xb = &xbuf[m.wv][m.lw * F::XS];
+    const cx<T> zero = mk(T(0), T(0));
+    cx<T> X[M][N2];
+    cx<T> v[N1];
+    T phi[N1];
+    if (phihat) {
+        const unsigned hbase = unsigned(lc) * unsigned(hpitch);
+        WDM_UNROLL
+        for (int i = 0; i < M; ++i)
+            WDM_UNROLL
+            for (int p = 0; p < N2; ++p) {
+                const int k = i * N2 + m.j + N1 * p;
+                cx<T> zz;
+                if (2 * k <= N) {
+                    zz = WDM_LDG(ok, phihat, hbase + unsigned(k));
+                } else {
+                    const cx<T> hh = WDM_LDG(ok, phihat, hbase + unsigned(N - k));
+                    zz = mk(hh.x, -hh.y);
+                }
+                X[i][p] = zz;
+            }
+        inv2<F, +1, PoissonPolicy>(X, v, m.j, xb, tw);
+        WDM_UNROLL
+        for (int kq = 0; kq < N1; ++kq) phi[kq] = v[kq].x;
+    } else {
+        WDM_UNROLL
+        for (int kq = 0; kq < N1; ++kq) phi[kq] = T(0);
+    }
+    const LineV lv = line_V(ka.vm, ka.ys[y], ka.zs[z]);
+    WDM_SCHED_FENCE();
+    WDM_UNROLL
+    for (int n1 = 0; n1 < N1; ++n1) v[n1] = WDM_LDG(ok, psi, rbase + unsigned(N2 * n1 + m.j));
+    double ms = 0.0, mx = 0.0, rp = 0.0, pm = 0.0, rv = 0.0;
+    WDM_UNROLL
+    for (int n1 = 0; n1 < N1; ++n1) {
+        const int x = N2 * n1 + m.j;
+        const double r = double(norm2(v[n1]));
+        ms += r;
+        mx = r > mx ? r : mx;
+        const double ph = double(phi[n1]);
+        rp += r * ph;
+        const double aph = __builtin_fabs(ph);
+        pm = aph > pm ? aph : pm;
+        if (ka.vm.flags) {
+            const double tx = double(x) * ka.xinv;
+            const double xc = (1.0 - tx) * ka.x0 + tx * ka.x1;
+            rv += r * eval_V_line<T>(ka.vm, lv, xc, unsigned(lc) * N + x, v[n1]);
+        }
+    }
+    fwd2<F, -1, P>(v, X, m.j, xb, tw);
+    double ks = 0.0;
+    WDM_UNROLL
+    for (int i = 0; i < M; ++i)
+        WDM_UNROLL
+        for (int k2 = 0; k2 < N2; ++k2) {
+            const int mi = fft_signed((i * N2 + m.j) + N1 * k2, N);
+            ks += double(mi * mi) * double(norm2(X[i][k2]));
+        }
+    if (!ok) {
+        ms = mx = rp = pm = rv = ks = 0.0;
+    }
+    double* rb = rbuf[m.wv];
+    const double s_ms = wave_tree(rb, m.lane, ms, false);
+    const double s_mx = wave_tree(rb, m.lane, mx, true);
+    const double s_rp = wave_tree(rb, m.lane, rp, false);
+    const double s_pm = wave_tree(rb, m.lane, pm, true);
+    const double s_rv = wave_tree(rb, m.lane, rv, false);
+    const double s_ks = wave_tree(rb, m.lane, ks, false);
+    if (m.lane == 0) {
+        double* o = partials + size_t(RS_COUNT) * size_t(m.w);
+        o[RS_MASS] = s_ms;
+        o[RS_RHOMAX] = s_mx;
+        o[RS_RHOPHI] = s_rp;
+        o[RS_PHIMAX] = s_pm;
+        o[RS_RHOV] = s_rv;
+        o[RS_KX] = s_ks * kscale;
     }
 }
 
-// sum rho*Phi and sum rho*V over the grid (W and V_ext numerators); rho from psi,
-// Phi from a real FP64 array.
-template <class T>
-WDM_KERNEL(256)
-void k_potential_sums(const cx<T>* __restrict__ psi, const double* __restrict__ phi, int nx, int ny, int nz,
-                      const double* __restrict__ xs, const double* __restrict__ ys, const double* __restrict__ zs,
-                      VModel vm, double* __restrict__ partials) {
-    WDM_SHARED double s1[256], s2[256];
-    const int t = int(threadIdx.x);
-    const size_t n = size_t(nx) * ny * nz;
-    double a = 0.0, b = 0.0;
-    for (size_t i = size_t(blockIdx.x) * 256 + t; i < n; i += size_t(gridDim.x) * 256) {
-        const int x = int(i % nx);
-        const int y = int((i / nx) % ny);
-        const int z = int(i / (size_t(nx) * ny));
-        const double r = double(norm2(psi[i]));
-        if (phi) a += r * phi[i];
-        if (vm.flags) b += r * eval_V<T>(vm, xs[x], ys[y], zs[z], unsigned(i), psi[i]);
-    }
-    s1[t] = a;
-    s2[t] = b;
-    __syncthreads();
-    if (t == 0) {
-        double A = 0.0, B = 0.0;
-        for (int i = 0; i < 256; ++i) { A += s1[i]; B += s2[i]; }
-        partials[2 * blockIdx.x] = A;
-        partials[2 * blockIdx.x + 1] = B;
-    }
+// y or z lines (column layout of k_col): sum k_line^2 |F_line psi|^2 per wave.
+template <class T, class P, class F, int NW>
+WDM_KERNEL(NW * 64)
+void k_col_red(const cx<T>* __restrict__ data, int ncols, int nother, unsigned sline, unsigned sother,
+               const typename TwEntry<T>::type* __restrict__ tw, double kscale, double* __restrict__ partials) {
+    constexpr int N1 = F::N1, N2 = F::N2, M = F::M, N = F::N;
+    WDM_SHARED cx<T> xbuf[NW][F::XBUF];
+    WDM_SHARED typename TwEntry<T>::type tw_s[F::N];
+    WDM_SHARED double rbuf[NW][kWave];
+    tw = stage_table<F::N>(tw_s, tw);
+    const LaneMap m = lane_map<F, NW>();
+    const int ntiles = (ncols + F::LPW - 1) / F::LPW;
+    const int o = m.w / ntiles, tile = m.w % ntiles;
+    const int c = tile * F::LPW + m.lw;
+    const bool ok = (o < nother) && (c < ncols);
+    const int oc = ok ? o : 0, cc = ok ? c : 0;
+    const unsigned base = unsigned(oc) * sother + unsigned(cc) + unsigned(m.j) * sline;
+    const unsigned step = unsigned(N2) * sline;
+    cx<T>* xb = &xbuf[m.wv][m.lw * F::XS];
+    const cx<T> zero = mk(T(0), T(0));
+    cx<T> v[N1];
+    cx<T> X[M][N2];
+    WDM_UNROLL
+    for (int n1 = 0; n1 < N1; ++n1) v[n1] = WDM_LDG(ok, data, base + unsigned(n1) * step);
+    fwd2<F, -1, P>(v, X, m.j, xb, tw);
+    double ks = 0.0;
+    WDM_UNROLL
+    for (int i = 0; i < M; ++i)
+        WDM_UNROLL
+        for (int k2 = 0; k2 < N2; ++k2) {
+            const int mi = fft_signed((i * N2 + m.j) + N1 * k2, N);
+            ks += double(mi * mi) * double(norm2(X[i][k2]));
+        }
+    if (!ok) ks = 0.0;
+    const double s = wave_tree(rbuf[m.wv], m.lane, ks, false);
+    if (m.lane == 0) partials[m.w] = s * kscale;
 }
 
 // Element-wise conversions between the plan precision and FP64/FP32 host layouts.

@@ -54,7 +54,18 @@ struct Comp {};
 constexpr double kTwoPi = 6.283185307179586476925286766559;
 constexpr double kRsqrt2 = 0.70710678118654752440084436210485;
 
+// The table generators below run on the host (plan tables) and on the device
+// (the allocation-free Poisson call builds its tables in place): with
+// contraction off both evaluate the same IEEE operations, so the tables are
+// bit-identical.
+#if defined(__clang__)
+#define WDM_NO_CONTRACT _Pragma("clang fp contract(off)")
+#else
+#define WDM_NO_CONTRACT
+#endif
+
 constexpr double taylor_sin(double t) {
+    WDM_NO_CONTRACT
     double t2 = t * t, term = t, s = t;
     for (int n = 1; n < 14; ++n) {
         term *= -t2 / double((2 * n) * (2 * n + 1));
@@ -63,6 +74,7 @@ constexpr double taylor_sin(double t) {
     return s;
 }
 constexpr double taylor_cos(double t) {
+    WDM_NO_CONTRACT
     double t2 = t * t, term = 1.0, c = 1.0;
     for (int n = 1; n < 14; ++n) {
         term *= -t2 / double((2 * n - 1) * (2 * n));
@@ -78,6 +90,7 @@ struct CS {
 // cos and sin of 2*pi*num/den, reduced to the first octant with exact
 // symmetries so that multiples of pi/4 come out exact.
 constexpr CS cs2pi(long num, long den) {
+    WDM_NO_CONTRACT
     long n = ((num % den) + den) % den;
     if ((8 * n) % den == 0) {
         const double r = kRsqrt2;
@@ -243,7 +256,7 @@ template <> struct TwEntry<float> { typedef TwF type; };
 // hi * x product is then inexact for general x, so one fixed fma order is
 // safe: the corrections ride on inexact products and the final fma rounds the
 // whole sum once (see is_pow2f).  No per-lane operand selects are needed.
-inline HL split_hl_table(double v) {
+WDM_HDFI HL split_hl_table(double v) {
     float h = float(v);
     if (is_pow2f(h)) {
         unsigned b = __builtin_bit_cast(unsigned, h);

@@ -94,6 +94,12 @@ struct AxisOps {
                    const AdjXArgs& a, double* partials, hipStream_t s);
     void (*adj_xg)(bool fwd, cx<T>* g, const cx<T>* psi_n, const cx<T>* uhat, int nlines, int hpitch, const TwE* tw,
                    const AdjXArgs& a, hipStream_t s);
+    // read-only reductions (k_x_red: RS_COUNT partials per launched wave;
+    // k_col_red: one per launched wave)
+    void (*x_red)(const cx<T>* psi, const cx<T>* phihat, int nlines, int hpitch, const TwE* tw, const KickArgs& ka,
+                  double kscale, double* partials, hipStream_t s);
+    void (*col_red)(const cx<T>* data, int ncols, int nother, unsigned sline, unsigned sother, const TwE* tw,
+                    double kscale, double* partials, hipStream_t s);
 };
 
 // Grid sizes (in workgroups of NW waves).
@@ -111,7 +117,7 @@ inline unsigned col_blocks(int ncols, int nother) {
 template <class T, class F, int NW = kNW<T>>
 struct Launch {
     typedef typename TwEntry<T>::type TwE;
-    typedef typename std::conditional<std::is_same<T, float>::value, Comp, Plain>::type PsiP;
+    typedef typename std::conditional<std::is_same<T, float>::value, PsiPolicyF32, Plain>::type PsiP;
 
     // staged variant: one workgroup per CU walking tpw tiles per wave
     template <class PP, int MODE, int KF>
@@ -245,9 +251,20 @@ struct Launch {
             WDM_LAUNCH((k_adj_xg<T, PsiP, F, false, NW>), dim3(x_blocks<NW, F>(nlines)), dim3(NW * 64), s, g, pn, uh,
                        nlines, hp, tw, a);
     }
+    static void x_red(const cx<T>* psi, const cx<T>* ph, int nlines, int hp, const TwE* tw, const KickArgs& ka,
+                      double kscale, double* partials, hipStream_t s) {
+        WDM_LAUNCH((k_x_red<T, PsiP, F, NW>), dim3(x_blocks<NW, F>(nlines)), dim3(NW * 64), s, psi, ph, nlines, hp, tw,
+                   ka, kscale, partials);
+    }
+    static void col_red(const cx<T>* d, int ncols, int nother, unsigned sl, unsigned so, const TwE* tw, double kscale,
+                        double* partials, hipStream_t s) {
+        WDM_LAUNCH((k_col_red<T, PsiP, F, NW>), dim3(col_blocks<NW, F>(ncols, nother)), dim3(NW * 64), s, d, ncols,
+                   nother, sl, so, tw, kscale, partials);
+    }
     static const AxisOps<T>* table() {
         static const AxisOps<T> ops = {F::N, F::N1, F::N2, F::LPW, NW, &col, &x_kick, &x_inv_dens,
-                                       &x_dens, &x_r2c_real, &x_c2r_real, &x_fft, &adj_xs, &adj_xg};
+                                       &x_dens, &x_r2c_real, &x_c2r_real, &x_fft, &adj_xs, &adj_xg,
+                                       &x_red, &col_red};
         return &ops;
     }
 };

@@ -76,8 +76,6 @@ double drift_k(int n, double X, int m) {
     return 2.0 * M_PI * f;
 }
 
-int fft_int(int n, int m) { return (2 * m < n) ? m : m - n; }
-
 }  // namespace
 
 struct AxisState {
@@ -112,6 +110,12 @@ struct PassTimer {
 namespace {
 struct AdjState;
 void adj_release(wdm_plan* p);
+}  // namespace
+struct TidalState;
+namespace {
+void tidal_release(wdm_plan* p);
+template <class T>
+void tidal_prepare(wdm_plan* p, double t);
 }  // namespace
 
 struct wdm_plan {
@@ -149,6 +153,8 @@ struct wdm_plan {
     double last_kick_t = 0.0;
     PassTimer* timer = nullptr;
     AdjState* adj = nullptr;  // adjoint tape (wdm_adjoint_*), created on first use
+    TidalState* tidal = nullptr;  // MW tidal table and trajectory (wdm_set_tidal_model)
+    TidalModel tidal_kick;        // the tidal term's constants at the last kick
     void* zspec_out = nullptr; // when set, the fused z pass also stores the 3-D spectrum here
     unsigned gen = 0;          // bumped by every change a captured step would not see
     unsigned long long psi_epoch = 0;  // bumped whenever psi is replaced or stepped
@@ -186,24 +192,33 @@ const AxisOps<T>* ops_of(const wdm_plan* p, int a) {
     return static_cast<const AxisOps<T>*>(p->ax[a].ops);
 }
 
+// Twiddle table of an axis of length n: W_n^j for j < n (FP64 {c, s}; FP32
+// {c_hi, s_hi, c_lo, s_lo} split by split_hl_table).  Host side, into h[0..n).
+template <class T>
+WDM_HDFI typename TwEntry<T>::type twiddle_entry(int j, int n) {
+    const CS v = cs2pi(j, n);
+    if constexpr (std::is_same<T, double>::value) {
+        return mk(v.c, v.s);
+    } else {
+        const HL c = split_hl_table(v.c), s = split_hl_table(v.s);
+        TwF w;
+        w.ch = c.h;
+        w.cl = c.l;
+        w.sh = s.h;
+        w.sl = s.l;
+        return w;
+    }
+}
+template <class T>
+void fill_twiddles(typename TwEntry<T>::type* h, int n) {
+    for (int j = 0; j < n; ++j) h[j] = twiddle_entry<T>(j, n);
+}
+
 template <class T>
 void build_twiddles(AxisState& a) {
     typedef typename TwEntry<T>::type TwE;
     std::vector<TwE> h(a.n);
-    for (int j = 0; j < a.n; ++j) {
-        const CS v = cs2pi(j, a.n);
-        if constexpr (std::is_same<T, double>::value) {
-            h[j] = mk(v.c, v.s);
-        } else {
-            const HL c = split_hl_table(v.c), s = split_hl_table(v.s);
-            TwF w;
-            w.ch = c.h;
-            w.cl = c.l;
-            w.sh = s.h;
-            w.sl = s.l;
-            h[j] = w;
-        }
-    }
+    fill_twiddles<T>(h.data(), a.n);
     TwE* d = a.tw.get<TwE>(a.n);
     hip_check(hipMemcpy(d, h.data(), sizeof(TwE) * a.n, hipMemcpyHostToDevice), "twiddle upload");
 }
@@ -235,21 +250,33 @@ void ensure_drift(wdm_plan* p, double dt) {
     p->hbar_cached = p->hbar_over_m;
 }
 
+// k^2 of the Poisson lattice (period 2X) for axis a of length n: the x axis
+// holds the half spectrum m = 0..n/2, y and z the FFT order.  Host side, into
+// h[0..k2_count(a, n)) in the plan precision T.
+WDM_HDFI int k2_count(int a, int n) { return a == 0 ? n / 2 + 1 : n; }
+template <class T>
+WDM_HDFI T k2_entry(int a, int n, int m, double X) {
+    WDM_NO_CONTRACT
+    const int q = (a == 0 || 2 * m < n) ? m : m - n;
+    const double k = 2.0 * M_PI * q / (2.0 * X);
+    return T(k * k);
+}
+template <class T>
+void fill_k2(T* h, int a, int n, double X) {
+    for (int m = 0; m < k2_count(a, n); ++m) h[m] = k2_entry<T>(a, n, m, X);
+}
+
 void build_k2(wdm_plan* p) {
     for (int a = 0; a < 3; ++a) {
-        const int n = p->ax[a].n;
-        const int cnt = (a == 0) ? n / 2 + 1 : n;
-        const double L = 2.0 * p->X[a];
-        std::vector<double> h(cnt);
-        for (int m = 0; m < cnt; ++m) {
-            const double k = 2.0 * M_PI * (a == 0 ? m : fft_int(n, m)) / L;
-            h[m] = k * k;
-        }
+        const int n = p->ax[a].n, cnt = k2_count(a, n);
         if (p->precision == WDM_F32) {
-            std::vector<float> f(h.begin(), h.end());
+            std::vector<float> f(cnt);
+            fill_k2<float>(f.data(), a, n, p->X[a]);
             float* d = p->ax[a].k2.get<float>(f.size());
             hip_check(hipMemcpy(d, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice), "k2 upload");
         } else {
+            std::vector<double> h(cnt);
+            fill_k2<double>(h.data(), a, n, p->X[a]);
             upload(p->ax[a].k2, h);
         }
     }
@@ -262,34 +289,44 @@ void launch_check(wdm_plan* p, const char* what) {
 }
 
 // ---- pass sequences ------------------------------------------------------
-template <class T>
-void poisson_chain(wdm_plan* p, double scale) {
-    // rhohat holds R2C_x(rho); run y FWD, z (FWD * scale/k^2 * INV), y INV.
-    const AxisOps<T>* oy = ops_of<T>(p, 1);
-    const AxisOps<T>* oz = ops_of<T>(p, 2);
-    typedef typename TwEntry<T>::type TwE;
-    cx<T>* rh = p->rhohat.as<cx<T>>();
-    const int nh = p->nx / 2 + 1, hp = p->hpitch;
+// The Poisson chain on a half spectrum that holds R2C_x(rho): y FWD, z (FWD *
+// -scale/(N k^2) * INV, k = 0 -> 0), y INV.  Buffer-level, shared by the plan
+// (poisson_chain) and the allocation-free entry point, so both issue the same
+// launches; hook(pass, before) brackets each launch (pass timers).
+template <class T, class Hook>
+void poisson_passes(const AxisOps<T>* oy, const AxisOps<T>* oz, int nx, int ny, int nz, int hp, cx<T>* rh,
+                    const typename TwEntry<T>::type* twy, const typename TwEntry<T>::type* twz, const void* k2x,
+                    const void* k2y, const void* k2z, double scale, hipStream_t s, Hook&& hook) {
+    const int nh = nx / 2 + 1;
+    const double ncell = double(nx) * double(ny) * double(nz);
     KTabs none = {nullptr, nullptr, nullptr, 0.0, nullptr};
-    if (p->ny > 1) {
-        arm(p, WDM_PASS_PYFWD);
-        oy->col(COL_FWD, KF_PNONE, rh, nh, p->nz, unsigned(hp), unsigned(hp * p->ny), p->ax[1].tw.as<TwE>(), none,
-                p->stream);
-        mark(p, WDM_PASS_PYFWD);
+    if (ny > 1) {
+        hook(WDM_PASS_PYFWD, true);
+        oy->col(COL_FWD, KF_PNONE, rh, nh, nz, unsigned(hp), unsigned(hp * ny), twy, none, s);
+        hook(WDM_PASS_PYFWD, false);
     }
     // Phi_k = -scale * rho_k / k^2, with the 1/N of the unnormalised inverse folded in
-    KTabs kt = {p->ax[0].k2.p, p->ax[1].k2.p, p->ax[2].k2.p,
-                -scale / double(p->ncell), nullptr};
-    arm(p, WDM_PASS_PZ);
-    oz->col(COL_FMI, KF_POISSON, rh, nh, p->ny, unsigned(hp * p->ny), unsigned(hp), p->ax[2].tw.as<TwE>(), kt,
-            p->stream);
-    mark(p, WDM_PASS_PZ);
-    if (p->ny > 1) {
-        arm(p, WDM_PASS_PYINV);
-        oy->col(COL_INV, KF_PNONE, rh, nh, p->nz, unsigned(hp), unsigned(hp * p->ny), p->ax[1].tw.as<TwE>(), none,
-                p->stream);
-        mark(p, WDM_PASS_PYINV);
+    KTabs kt = {k2x, k2y, k2z, -scale / ncell, nullptr};
+    hook(WDM_PASS_PZ, true);
+    oz->col(COL_FMI, KF_POISSON, rh, nh, ny, unsigned(hp * ny), unsigned(hp), twz, kt, s);
+    hook(WDM_PASS_PZ, false);
+    if (ny > 1) {
+        hook(WDM_PASS_PYINV, true);
+        oy->col(COL_INV, KF_PNONE, rh, nh, nz, unsigned(hp), unsigned(hp * ny), twy, none, s);
+        hook(WDM_PASS_PYINV, false);
     }
+}
+
+template <class T>
+void poisson_chain(wdm_plan* p, double scale) {
+    // rhohat holds R2C_x(rho)
+    typedef typename TwEntry<T>::type TwE;
+    poisson_passes<T>(ops_of<T>(p, 1), ops_of<T>(p, 2), p->nx, p->ny, p->nz, p->hpitch, p->rhohat.as<cx<T>>(),
+                      p->ax[1].tw.as<TwE>(), p->ax[2].tw.as<TwE>(), p->ax[0].k2.p, p->ax[1].k2.p, p->ax[2].k2.p,
+                      scale, p->stream, [p](int pass, bool before) {
+                          if (before) arm(p, pass);
+                          else mark(p, pass);
+                      });
     launch_check<T>(p, "poisson chain");
 }
 
@@ -363,6 +400,11 @@ VModel make_vmodel(const wdm_plan* p, double t) {
         vm.q_c[2] = q[12];
     }
     vm.kappa = p->kappa;
+    vm.gnx = p->nx;
+    vm.gny = p->ny;
+    vm.tidal = p->tidal_kick;
+    // the tidal term exists once a kick has computed its constants
+    if ((vm.flags & V_TIDAL) && !p->tidal_kick.table) vm.flags &= ~V_TIDAL;
     return vm;
 }
 
@@ -408,6 +450,7 @@ template <class T>
 void x_kick(wdm_plan* p, bool fwd, double dt_kick, double t) {
     typedef typename TwEntry<T>::type TwE;
     const AxisOps<T>* ox = ops_of<T>(p, 0);
+    if (p->vflags & V_TIDAL) tidal_prepare<T>(p, t);  // this kick's peak, moments and trajectory offset
     const KickArgs ka = make_kick<T>(p, dt_kick, t);
     arm(p, WDM_PASS_XKICK);
     ox->x_kick(fwd, p->psi.as<cx<T>>(), p->rhohat.as<cx<T>>(), p->ny * p->nz, p->hpitch, p->ax[0].tw.as<TwE>(), ka,
@@ -530,23 +573,6 @@ void do_step(wdm_plan* p, int nsteps, double dt, double dt_kick, double t0, int 
 }
 
 template <class T>
-void forward3d(wdm_plan* p, cx<T>* buf) {
-    typedef typename TwEntry<T>::type TwE;
-    const AxisOps<T>* ox = ops_of<T>(p, 0);
-    const AxisOps<T>* oy = ops_of<T>(p, 1);
-    const AxisOps<T>* oz = ops_of<T>(p, 2);
-    KTabs none = {nullptr, nullptr, nullptr, 0.0, nullptr};
-    ox->x_fft(-1, buf, p->ny * p->nz, p->ax[0].tw.as<TwE>(), p->stream);
-    if (p->ny > 1)
-        oy->col(COL_FWD, KF_NONE, buf, p->nx, p->nz, unsigned(p->nx), unsigned(p->nx * p->ny),
-                p->ax[1].tw.as<TwE>(), none, p->stream);
-    if (p->nz > 1)
-        oz->col(COL_FWD, KF_NONE, buf, p->nx, p->ny, unsigned(p->nx * p->ny), unsigned(p->nx),
-                p->ax[2].tw.as<TwE>(), none, p->stream);
-    hip_check(hipGetLastError(), "forward 3-D FFT");
-}
-
-template <class T>
 void phi_real(wdm_plan* p, double* dst_dev) {
     typedef typename TwEntry<T>::type TwE;
     ensure_phi<T>(p);
@@ -596,32 +622,6 @@ void k_reduce_partials(const double* __restrict__ partials, int n, int second_su
 }
 
 unsigned red_blocks(size_t n) { return unsigned(std::min<size_t>((n + 255) / 256, 1024)); }
-
-// generic grid reductions ---------------------------------------------------
-WDM_KERNEL(256)
-void k_abs_max_sum(const double* __restrict__ a, size_t n, double* __restrict__ partials) {
-    WDM_SHARED double s1[256], s2[256];
-    const int t = int(threadIdx.x);
-    double s = 0.0, mx = 0.0;
-    for (size_t i = size_t(blockIdx.x) * 256 + t; i < n; i += size_t(gridDim.x) * 256) {
-        const double v = a[i];
-        s += v;
-        const double av = __builtin_fabs(v);
-        mx = av > mx ? av : mx;
-    }
-    s1[t] = s;
-    s2[t] = mx;
-    __syncthreads();
-    if (t == 0) {
-        double S = 0.0, MX = 0.0;
-        for (int i = 0; i < 256; ++i) {
-            S += s1[i];
-            MX = s2[i] > MX ? s2[i] : MX;
-        }
-        partials[2 * blockIdx.x] = S;
-        partials[2 * blockIdx.x + 1] = MX;
-    }
-}
 
 template <class T>
 WDM_KERNEL(256)
@@ -792,66 +792,122 @@ void ledger_finish(wdm_plan* p) {
     p->phi_valid = false;
 }
 
+// Fixed-order final reduction of per-wave slot partials: block s reduces slot s
+// of nwaves records of nslots doubles (Neumaier-compensated sum, or max when
+// bit s of max_mask is set) into out[s].
+WDM_KERNEL(256)
+void k_reduce_slots(const double* __restrict__ partials, int nwaves, int nslots, unsigned max_mask,
+                    double* __restrict__ out) {
+    WDM_SHARED double ssum[256], scmp[256];
+    const int t = int(threadIdx.x), slot = int(blockIdx.x);
+    const bool is_max = (max_mask >> slot) & 1u;
+    double sum = 0.0, comp = 0.0;
+    for (int i = t; i < nwaves; i += 256) {
+        const double v = partials[size_t(i) * size_t(nslots) + size_t(slot)];
+        if (is_max) {
+            sum = v > sum ? v : sum;
+        } else {
+            const double tt = sum + v;
+            if (__builtin_fabs(sum) >= __builtin_fabs(v)) comp += (sum - tt) + v;
+            else comp += (v - tt) + sum;
+            sum = tt;
+        }
+    }
+    ssum[t] = sum;
+    scmp[t] = comp;
+    __syncthreads();
+    if (t == 0) {
+        double S = 0.0, C = 0.0;
+        for (int i = 0; i < 256; ++i) {
+            const double v = ssum[i];
+            if (is_max) {
+                S = v > S ? v : S;
+            } else {
+                const double tt = S + v;
+                if (__builtin_fabs(S) >= __builtin_fabs(v)) C += (S - tt) + v;
+                else C += (v - tt) + S;
+                S = tt;
+                C += scmp[i];
+            }
+        }
+        out[slot] = is_max ? S : S + C;
+    }
+}
+
+// waves launched by an x pass / a column pass of a table
+template <class T>
+size_t x_waves(const AxisOps<T>* o, int nlines) {
+    const size_t w = (size_t(nlines) + o->lpw - 1) / o->lpw;
+    return (w + o->nw - 1) / o->nw * o->nw;
+}
+template <class T>
+size_t col_waves(const AxisOps<T>* o, int ncols, int nother) {
+    const size_t w = (size_t(ncols) + o->lpw - 1) / o->lpw * size_t(nother);
+    return (w + o->nw - 1) / o->nw * o->nw;
+}
+
+// Diagnostics of the current psi by read-only passes (no N-sized scratch):
+// k_x_red for mass, max rho, sum rho Phi, max |Phi|, sum rho V and the kx part
+// of the kinetic sum; k_col_red for the ky and kz parts.  Phi is the plan's
+// Phi_hat (computed first when psi changed), V the models at the last kick.
 template <class T>
 void reduce_all(wdm_plan* p, double* out, int nout) {
+    typedef typename TwEntry<T>::type TwE;
     if (p->kspace) fail(WDM_ERR_STATE, "wdm_reduce: psi is in k-space");
     double r[WDM_RED_COUNT] = {0};
     const double dV = p->dx[0] * p->dx[1] * p->dx[2];
-    double ms[2];
-    psi_sums<T>(p, ms);
-    r[WDM_RED_MASS] = ms[0] * dV;
-    r[WDM_RED_PSIMAX] = std::sqrt(ms[1]);
+    const bool sg = p->self_gravity;
+    if (sg && nout > WDM_RED_WSELF) ensure_phi<T>(p);
+    const AxisOps<T>* ox = ops_of<T>(p, 0);
+    const int nlines = p->ny * p->nz;
+    KickArgs ka = make_kick<T>(p, 0.0, p->last_kick_t);
+    ka.norm = nullptr;
+    const double kx0 = 2.0 * M_PI / (2.0 * p->X[0]);
+    double* part = p->partials.as<double>();
+    double* o = p->redout.as<double>();
+    ox->x_red(p->psi.as<cx<T>>(), (sg && nout > WDM_RED_WSELF) ? p->rhohat.as<cx<T>>() : nullptr, nlines,
+              p->hpitch, p->ax[0].tw.as<TwE>(), ka, kx0 * kx0, part, p->stream);
+    hip_check(hipGetLastError(), "x reduction");
+    const unsigned maxmask = (1u << RS_RHOMAX) | (1u << RS_PHIMAX);
+    WDM_LAUNCH((k_reduce_slots), dim3(RS_COUNT), dim3(256), p->stream, (const double*)part,
+               int(x_waves(ox, nlines)), int(RS_COUNT), maxmask, o);
+    double h[RS_COUNT + 2] = {0};
+    size_t nh = RS_COUNT;
     if (nout > WDM_RED_KINETIC) {
-        cx<T>* sc = p->scratch.get<cx<T>>(p->ncell);
-        hip_check(hipMemcpyAsync(sc, p->psi.as<cx<T>>(), sizeof(cx<T>) * p->ncell, hipMemcpyDeviceToDevice,
-                                 p->stream),
-                  "copy");
-        forward3d<T>(p, sc);
-        // full-length k^2 tables (FFT order on every axis)
-        std::vector<double> hk[3];
-        DBuf kfull[3];
-        for (int a = 0; a < 3; ++a) {
+        // ky and kz parts (their partials follow the x slots' final values in redout)
+        for (int a = 1; a < 3; ++a) {
             const int n = p->ax[a].n;
-            hk[a].resize(n);
-            for (int m = 0; m < n; ++m) {
-                const double k = 2.0 * M_PI * fft_int(n, m) / (2.0 * p->X[a]);
-                hk[a][m] = k * k;
+            if (n == 1) {  // no k != 0 on this axis: its slot is an empty sum
+                WDM_LAUNCH((k_reduce_slots), dim3(1), dim3(256), p->stream, (const double*)part, 0, 1, 0u,
+                           o + RS_COUNT + (a - 1));
+                continue;
             }
-            upload(kfull[a], hk[a]);
+            const AxisOps<T>* oa = ops_of<T>(p, a);
+            const double ka0 = 2.0 * M_PI / (2.0 * p->X[a]);
+            const unsigned sl = a == 1 ? unsigned(p->nx) : unsigned(p->nx * p->ny);
+            const unsigned so = a == 1 ? unsigned(p->nx * p->ny) : unsigned(p->nx);
+            const int nother = a == 1 ? p->nz : p->ny;
+            oa->col_red(p->psi.as<cx<T>>(), p->nx, nother, sl, so, p->ax[a].tw.as<TwE>(), ka0 * ka0, part, p->stream);
+            hip_check(hipGetLastError(), "column reduction");
+            WDM_LAUNCH((k_reduce_slots), dim3(1), dim3(256), p->stream, (const double*)part,
+                       int(col_waves(oa, p->nx, nother)), 1, 0u, o + RS_COUNT + (a - 1));
         }
-        const unsigned nb = red_blocks(p->ncell);
-        p->partials.get<double>(std::max<size_t>(p->npartials, 2 * nb));
-        WDM_LAUNCH((k_spectral_sums<T>), dim3(nb), dim3(256), p->stream, sc, p->nx, p->ny, p->nz,
-                   kfull[0].as<double>(), kfull[1].as<double>(), kfull[2].as<double>(), p->partials.as<double>());
-        hip_check(hipGetLastError(), "spectral sums");
-        double ks[2];
-        finalize(p, int(nb), ks, true);
-        for (auto& b : kfull) b.reset();
-        r[WDM_RED_KINETIC] = 0.5 * p->hbar_over_m * p->hbar_over_m * ks[0] * dV / double(p->ncell);
+        nh = RS_COUNT + 2;
+    }
+    hip_check(hipGetLastError(), "reduce slots");
+    hip_check(hipMemcpyAsync(h, o, sizeof(double) * nh, hipMemcpyDeviceToHost, p->stream), "reduce copy");
+    hip_check(hipStreamSynchronize(p->stream), "reduce sync");
+    r[WDM_RED_MASS] = h[RS_MASS] * dV;
+    r[WDM_RED_PSIMAX] = std::sqrt(h[RS_RHOMAX]);
+    if (nout > WDM_RED_KINETIC) {
+        // sum_k k^2 |psi_hat|^2 / N = sum_a S_a / n_a (S_a: the axis-a pass's sum)
+        const double sk = h[RS_KX] / double(p->nx) + h[RS_COUNT] / double(p->ny) + h[RS_COUNT + 1] / double(p->nz);
+        r[WDM_RED_KINETIC] = 0.5 * p->hbar_over_m * p->hbar_over_m * sk * dV;
     }
     if (nout > WDM_RED_WSELF) {
-        double* phi = p->scratch_real.get<double>(p->ncell);
-        const bool sg = p->self_gravity;
-        if (sg) phi_real<T>(p, phi);
-        const unsigned nb = red_blocks(p->ncell);
-        p->partials.get<double>(std::max<size_t>(p->npartials, 2 * nb));
-        VModel vm = make_vmodel(p, p->last_kick_t);
-        WDM_LAUNCH((k_potential_sums<T>), dim3(nb), dim3(256), p->stream, p->psi.as<cx<T>>(),
-                   sg ? (const double*)phi : nullptr, p->nx, p->ny, p->nz, p->ax[0].coord.as<double>(),
-                   p->ax[1].coord.as<double>(), p->ax[2].coord.as<double>(), vm, p->partials.as<double>());
-        hip_check(hipGetLastError(), "potential sums");
-        double ws[2];
-        finalize(p, int(nb), ws, true);
-        r[WDM_RED_WSELF] = 0.5 * ws[0] * dV;
-        r[WDM_RED_VEXT] = ws[1] * dV;
-        if (nout > WDM_RED_PHIMAX && sg) {
-            WDM_LAUNCH((k_abs_max_sum), dim3(nb), dim3(256), p->stream, (const double*)phi, p->ncell,
-                       p->partials.as<double>());
-            hip_check(hipGetLastError(), "phi max");
-            double pm[2];
-            finalize(p, int(nb), pm);
-            r[WDM_RED_PHIMAX] = pm[1];
-        }
+        r[WDM_RED_WSELF] = sg ? 0.5 * h[RS_RHOPHI] * dV : 0.0;
+        r[WDM_RED_VEXT] = h[RS_RHOV] * dV;
+        r[WDM_RED_PHIMAX] = sg ? h[RS_PHIMAX] : 0.0;
     }
     for (int i = 0; i < nout && i < WDM_RED_COUNT; ++i) out[i] = r[i];
 }
@@ -1020,9 +1076,14 @@ void poisson_api(wdm_plan* p, const double* rho, double* phi_out, double scale) 
 template <class T>
 void update_partials(wdm_plan* p) {
     const AxisOps<T>* ox = ops_of<T>(p, 0);
-    const size_t waves = (size_t(p->ny) * p->nz + ox->lpw - 1) / ox->lpw;
-    p->npartials = 2 * ((waves + ox->nw - 1) / ox->nw) * ox->nw;
-    p->partials.get<double>(std::max<size_t>(p->npartials, 2 * 1024));
+    const size_t xw = x_waves(ox, p->ny * p->nz);
+    p->npartials = 2 * xw;
+    // capacity: the step's mass pairs, the reductions' slots (k_x_red,
+    // k_col_red of either column axis) and the diagnostics' blocks
+    size_t cap = std::max<size_t>(size_t(RS_COUNT) * xw, 2 * 1024);
+    if (p->ax[1].ops) cap = std::max(cap, col_waves(ops_of<T>(p, 1), p->nx, p->nz));
+    if (p->ax[2].ops) cap = std::max(cap, col_waves(ops_of<T>(p, 2), p->nx, p->ny));
+    p->partials.get<double>(cap);
 }
 
 template <class T>
@@ -1042,7 +1103,7 @@ void init_plan(wdm_plan* p) {
     p->hpitch = ((p->nx / 2 + 1) + WDM_HPITCH_ALIGN - 1) / WDM_HPITCH_ALIGN * WDM_HPITCH_ALIGN;
     p->rhohat.get<cx<T>>(size_t(p->hpitch) * p->ny * p->nz);
     update_partials<T>(p);
-    p->redout.get<double>(2);
+    p->redout.get<double>(RS_COUNT + 2);
 }
 
 int fail_code(wdm_plan* p, int code, const std::string& msg) {
@@ -1179,6 +1240,7 @@ int wdm_plan_destroy(wdm_plan* p) {
     if (!p) return WDM_OK;
     if (p->stream) hipStreamSynchronize(p->stream);
     adj_release(p);
+    tidal_release(p);
     for (auto& a : p->ax) {
         a.tw.reset();
         a.coord.reset();
@@ -1349,6 +1411,7 @@ int wdm_clear_potential_models(wdm_plan* p) {
     if (p) ++p->gen;
     if (!p) return fail_code(nullptr, WDM_ERR_ARG, "plan is NULL");
     p->vflags &= V_STATIC;
+    tidal_release(p);
     return WDM_OK;
 }
 
@@ -1466,3 +1529,4 @@ int wdm_synchronize(wdm_plan* p) {
 #include "wdm_ensemble.inl"
 #include "wdm_diag.inl"
 #include "wdm_ic.inl"
+#include "wdm_tidal.inl"

@@ -137,3 +137,56 @@ def soliton_superposition(shape, xmax, params, hbar_over_m, consistent=False) ->
         phase = ph + k[0] * x[:, None, None] + k[1] * y[None, :, None] + k[2] * z[None, None, :]
         psi += np.sqrt(soliton_density(r, rhoc, rc)) * np.exp(1j * phase)
     return np.asfortranarray(psi)
+
+
+# ---------------------------------------------------------------- acceleration -> velocity
+def compute_acceleration_field(plan, rho_halo, want=("phi", "ax", "ay", "az", "a_cdm")):
+    """compute_acceleration_field + generate_velocity_field's magnitude on the
+    device (src/ic/generation.jl:246-291, periodic gravity): Phi = 4 pi G
+    lap^-1 (rho_halo - mean) on the plan's lattice, a = -grad Phi (central
+    differences), a_CDM = |a| on the interior, 0 on the faces and at the
+    1-based centre cell.  Returns a dict of the requested host arrays."""
+    import ctypes as C
+    from . import _capi as A
+    rho = np.ascontiguousarray(np.asarray(rho_halo, dtype=np.float64).ravel(order="F"))
+    if rho.size != plan.ncell:
+        raise ValueError(f"rho_halo must have shape {plan.shape}")
+    names = ("phi", "ax", "ay", "az", "a_cdm")
+    outs = {k: np.empty(plan.ncell, dtype=np.float64) for k in names if k in want}
+    ptr = lambda a: a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None  # noqa: E731
+    rc = plan.lib.wdm_ic_acceleration(plan._h, ptr(rho), *[ptr(outs.get(k)) for k in names])
+    A.check(plan.lib, plan._h, rc)
+    return {k: v.reshape(plan.shape, order="F") for k, v in outs.items()}
+
+
+def zero_center_velocity(vx, vy, vz, bulk_center_size):
+    """adjust_velocity_field's bulk_center_size step (src/ic/generation.jl:316-326):
+    zero the velocity on the 1-based ranges div(N,2)-div(s,2)+1 :
+    div(N,2)-div(s,2)+s of every axis.  Host arrays, returned as copies."""
+    out = [np.array(v, dtype=np.float64, copy=True) for v in (vx, vy, vz)]
+    s = int(bulk_center_size)
+    if s > 0:
+        sl = tuple(slice(n // 2 - s // 2, n // 2 - s // 2 + s) for n in out[0].shape)
+        for v in out:
+            v[sl] = 0.0
+    return tuple(out)
+
+
+def velocity_ic(plan, rho_halo, velocity, bulk_center_size=0, bulk_perturb=False):
+    """The reference's IC chain from a halo density (src/ic/generation.jl:179-187,
+    303-337) on the device: accelerations and a_CDM, the velocity field
+    ``velocity(x, y, z, a_CDM) -> (vx, vy, vz)`` (the AstroIC velocity formulas
+    the reference calls are not in this image: the caller supplies them),
+    centre zeroing, removal of the mass-weighted mean velocity, the line
+    integrals of solve_vector_equation and psi = sqrt(rho) exp(i theta), left
+    in ``plan``.  ``bulk_perturb`` (the reference's zoom/shrink filters, also
+    absent) is not available."""
+    if bulk_perturb:
+        raise NotImplementedError("bulk_perturb needs the zoom/shrink filters, which are not in this image")
+    acc = compute_acceleration_field(plan, rho_halo, want=("a_cdm",))
+    x, y, z = grid_coords(plan.shape, plan.xmax, plan.consistent_grid)
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+    vx, vy, vz = velocity(X, Y, Z, acc["a_cdm"])
+    vx, vy, vz = zero_center_velocity(vx, vy, vz, bulk_center_size)
+    plan.set_ic_velocity(rho_halo, vx, vy, vz, remove_bulk=True)
+    return plan

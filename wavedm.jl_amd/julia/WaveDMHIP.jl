@@ -1,17 +1,15 @@
 # WaveDMHIP.jl -- Julia binding of libwavedm_hip.so (include/wavedm.h).
 #
-# A `:hip` backend for WaveDM.jl's solver step: the functions below keep the
-# names and argument meaning of the reference API and forward to the C ABI
-# with `ccall`.  Arrays cross as `pointer(A)` (column-major, x fastest,
-# interleaved complex), exactly the layout the library expects.
-#
-# Reference interfaces mirrored:
-#   apply_kick_step!                 src/solver/KDK.jl:37-77
-#   apply_drift_step!                src/solver/KDK.jl:88-101
-#   compute_gravitational_potential  src/solver/gravity.jl:7-35
-#   parallel_poisson                 src/parallel_poisson.jl:36-68
-#   compute_timestep                 src/physics/poisson.jl:35-50
-#   the loop body                    src/simulation_main.jl:536-551
+# `ccall` wrappers over the C ABI (one `HIPPlan` = one device-resident box),
+# plus, in WaveDMHIPBackend.jl (included at the end), methods of WaveDM.jl's
+# own solver-step functions at the reference signatures for a
+# `HIPDeviceConfig`: apply_kick_step! (src/solver/KDK.jl:37-77),
+# apply_drift_step! (:88-101), compute_gravitational_potential
+# (src/solver/gravity.jl:7-35), parallel_poisson (src/parallel_poisson.jl:36-48)
+# and compute_averaged_fields (src/io/save.jl:79-95).  The plan-level wrappers
+# here have names of their own (kick!, drift!, poisson, potential, step!) so
+# they never shadow WaveDM's.  Arrays cross as `pointer(A)` (column-major, x
+# fastest, interleaved complex), exactly the layout the library expects.
 #
 # This file is not executed in this repository (the image has no Julia); the
 # Python harness wavedm.jl_amd/kdk.py issues the same C-ABI call sequence and
@@ -21,10 +19,9 @@ module WaveDMHIP
 using ChainRulesCore
 
 export HIPPlan, ssfm, set_psi!, get_psi, set_border!, set_static_potential!, set_hernquist!,
-       set_quadrupole!, set_kappa!, parallel_poisson, compute_gravitational_potential,
-       apply_kick_step!, apply_drift_step!, step!, reduce_diagnostics, cfl_limits, step_properties,
-       accelerations, density_accumulate!, density_average, density_reset!, set_mass_ledger!,
-       set_ic_velocity!, baryon_mesh_potential
+       set_quadrupole!, set_kappa!, poisson, potential, kick!, drift!, step!, reduce_diagnostics,
+       split_energy, cfl_limits, step_properties, accelerations, density_accumulate!, density_average,
+       density_reset!, set_mass_ledger!, set_ic_velocity!, baryon_mesh_potential
 
 const libwdm = get(ENV, "WAVEDM_HIP_LIB", joinpath(@__DIR__, "..", "lib", "libwavedm_hip.so"))
 
@@ -70,6 +67,7 @@ mutable struct HIPPlan
     h::Ptr{Cvoid}
     dims::NTuple{3,Int}
     precision::Symbol
+    tape::Int  # counts taped forward runs (ssfm); a pullback checks it is still its own
     function HIPPlan(Nx::Integer, Ny::Integer, Nz::Integer, Xmax::Real, Ymax::Real, Zmax::Real;
                      precision::Symbol=:f64, consistent_grid::Bool=false,
                      hbar_over_m::Real=1.0, four_pi_G::Real=4π)
@@ -80,7 +78,7 @@ mutable struct HIPPlan
                    (Ref{Ptr{Cvoid}}, Cint, Cint, Cint, Cdouble, Cdouble, Cdouble, Cint, Cuint),
                    r, Nx, Ny, Nz, Xmax, Ymax, Zmax, prec, flags)
         check(rc, C_NULL)
-        p = new(r[], (Int(Nx), Int(Ny), Int(Nz)), precision)
+        p = new(r[], (Int(Nx), Int(Ny), Int(Nz)), precision, 0)
         finalizer(destroy!, p)
         check(ccall((:wdm_set_coeffs, libwdm), Cint, (Ptr{Cvoid}, Cdouble, Cdouble), p.h, hbar_over_m,
                     four_pi_G), p.h)
@@ -138,14 +136,14 @@ set_quadrupole!(p::HIPPlan; T0::AbstractMatrix, axis=(0.0, 0.0, 1.0), period=1.0
 set_kappa!(p::HIPPlan, κ::Real) = _model!(p, WDM_V_KAPPA, Float64[κ])
 
 """
-    parallel_poisson(p::HIPPlan, ρ::Array{Float64,3}; scale=1.0) -> Φ
+    poisson(p::HIPPlan, ρ::Array{Float64,3}; scale=1.0) -> Φ
 
 Periodic ∇²Φ = scale·(ρ - ⟨ρ⟩) on the plan's lattice (k = 2πm/(2Xmax), k=0 → 0),
-the `Periodic()` branch of src/parallel_poisson.jl:36-68.  The reference
-returns a complex array whose imaginary part is round-off; this returns the
-real part.
+the `Periodic()` arithmetic of src/parallel_poisson.jl:54-57.  Returns the real part (the
+reference's complex output has a round-off imaginary part).  `scale = 0` takes the plan's
+4πG.
 """
-function parallel_poisson(p::HIPPlan, ρ::Array{Float64,3}; scale::Real=1.0)
+function poisson(p::HIPPlan, ρ::Array{Float64,3}; scale::Real=1.0)
     size(ρ) == p.dims || throw(DimensionMismatch("ρ is $(size(ρ)), plan is $(p.dims)"))
     Φ = Array{Float64,3}(undef, p.dims)
     check(ccall((:wdm_poisson, libwdm), Cint, (Ptr{Cvoid}, Ptr{Cdouble}, Ptr{Cdouble}, Cdouble),
@@ -154,47 +152,37 @@ function parallel_poisson(p::HIPPlan, ρ::Array{Float64,3}; scale::Real=1.0)
 end
 
 """
-    baryon_mesh_potential(p::HIPPlan, ρ_baryon; four_pi_G=0.0) -> Φ_b
+    baryon_mesh_potential(p::HIPPlan, ρ_baryon; four_pi_G=nothing) -> Φ_b
 
 Φ_b = 4πG ∇⁻²(ρ_baryon - ⟨ρ_baryon⟩) for `baryon_mode = :mesh`
 (src/simulation_main.jl:244) on the plan's lattice and precision, real part.
-`four_pi_G = 0` takes the plan's coefficient.  Install it with
-`set_static_potential!(p, Φ_b)` so every kick adds it (src/solver/gravity.jl:24-28).
+`four_pi_G = nothing` takes the plan's coefficient; an explicit 0 gives Φ_b = 0.  Install
+it with `set_static_potential!(p, Φ_b)` so every kick adds it (src/solver/gravity.jl:24-28).
 """
-baryon_mesh_potential(p::HIPPlan, ρ_baryon::Array{Float64,3}; four_pi_G::Real=0.0) =
-    parallel_poisson(p, ρ_baryon; scale=four_pi_G)
+function baryon_mesh_potential(p::HIPPlan, ρ_baryon::Array{Float64,3}; four_pi_G::Union{Nothing,Real}=nothing)
+    four_pi_G === nothing && return poisson(p, ρ_baryon; scale=0.0)  # 0: the plan's coefficient
+    iszero(four_pi_G) && return zeros(Float64, p.dims)
+    return poisson(p, ρ_baryon; scale=four_pi_G)
+end
 
 """
-    compute_gravitational_potential(p::HIPPlan; Φ_b=nothing) -> (Φ_all, Φ_WaveDM)
+    potential(p::HIPPlan; Φ_b=nothing) -> (Φ_all, Φ_WaveDM)
 
 Φ_WaveDM = 4πG ∇⁻²(|ψ|² - ⟨|ψ|²⟩) of the plan's ψ (src/solver/gravity.jl:13-23);
 Φ_all adds the mesh baryon potential `Φ_b` when one is given (gravity.jl:24-28).
 """
-function compute_gravitational_potential(p::HIPPlan; Φ_b::Union{Nothing,Array{Float64,3}}=nothing)
+function potential(p::HIPPlan; Φ_b::Union{Nothing,Array{Float64,3}}=nothing)
     Φ = Array{Float64,3}(undef, p.dims)
     check(ccall((:wdm_get_phi, libwdm), Cint, (Ptr{Cvoid}, Ptr{Cdouble}), p.h, Φ), p.h)
     return (Φ_b === nothing ? Φ : Φ .+ Φ_b), Φ
 end
 
-"""
-    apply_kick_step!(p::HIPPlan, dt_kick, t; return_phi=false)
+"ψ ← FFT[exp(-i dt_kick (Φ[ψ] + V(t))) ψ] on the device (the kick of src/solver/KDK.jl:37-77); the plan keeps the spectrum."
+kick!(p::HIPPlan, dt_kick::Real, t::Real=0.0) =
+    (check(ccall((:wdm_kick, libwdm), Cint, (Ptr{Cvoid}, Cdouble, Cdouble), p.h, dt_kick, t), p.h); p)
 
-ψ ← FFT[exp(-i dt_kick (Φ[ψ] + V(t))) ψ] on the device (src/solver/KDK.jl:37-77).
-The plan keeps the spectrum (the reference's `spec`).  With `return_phi=true`
-the host Φ_all / Φ_WaveDM pair of the reference's return value is produced as
-well (one extra device-to-host copy).
-"""
-function apply_kick_step!(p::HIPPlan, dt_kick::Real, t::Real=0.0; return_phi::Bool=false)
-    Φ = return_phi ? first(compute_gravitational_potential(p)) : nothing
-    check(ccall((:wdm_kick, libwdm), Cint, (Ptr{Cvoid}, Cdouble, Cdouble), p.h, dt_kick, t), p.h)
-    return Φ, Φ, p
-end
-
-"ψ ← border .* IFFT[exp(-i k² dt/2) ψ_k] (src/solver/KDK.jl:88-101); `spec` is the plan."
-function apply_drift_step!(p::HIPPlan, dt::Real)
-    check(ccall((:wdm_drift, libwdm), Cint, (Ptr{Cvoid}, Cdouble), p.h, dt), p.h)
-    return p
-end
+"ψ ← border .* IFFT[exp(-i k² dt/2) ψ_k] (the drift of src/solver/KDK.jl:88-101)."
+drift!(p::HIPPlan, dt::Real) = (check(ccall((:wdm_drift, libwdm), Cint, (Ptr{Cvoid}, Cdouble), p.h, dt), p.h); p)
 
 """
     step!(p::HIPPlan, nsteps, dt; dt_kick=dt/2, t0=0.0, split=:reference)
@@ -211,12 +199,22 @@ function step!(p::HIPPlan, nsteps::Integer, dt::Real; dt_kick::Real=dt / 2, t0::
     return p
 end
 
-"(mass, kinetic, w_self, v_ext, psi_max, phi_max) of the plan's ψ, FP64."
+"(mass, kinetic, w_self, v_ext, psi_max, phi_max) of the plan's ψ, FP64 (read-only device passes)."
 function reduce_diagnostics(p::HIPPlan)
     out = zeros(Float64, WDM_RED_COUNT)
     check(ccall((:wdm_reduce, libwdm), Cint, (Ptr{Cvoid}, Ptr{Cdouble}, Cint), p.h, out, WDM_RED_COUNT), p.h)
     return (mass=out[1], kinetic=out[2], w_self=out[3], v_ext=out[4], psi_max=out[5], phi_max=out[6])
 end
+
+"""
+    split_energy(r, dt, dt_kick; split=:reference)
+
+The energy the loop's split conserves, from `reduce_diagnostics`: one kick of `dt_kick`
+per drift of `dt` (src/simulation_main.jl:538-547) advances H = K + (dt_kick/dt)(W + V_ext);
+the Strang split kicks for a whole dt per step, H = K + W + V_ext.
+"""
+split_energy(r, dt::Real, dt_kick::Real; split::Symbol=:reference) =
+    r.kinetic + (split === :strang ? 1.0 : dt_kick / dt) * (r.w_self + r.v_ext)
 
 """
     step_properties(p::HIPPlan) -> NamedTuple
@@ -298,16 +296,22 @@ function ssfm(p::HIPPlan, ψ0::Array{ComplexF64,3}, θ::NTuple{5,Float64}, nstep
     set_psi!(p, ψ0)
     check(ccall((:wdm_adjoint_forward, libwdm), Cint, (Ptr{Cvoid}, Cint, Cdouble, Cdouble, Cdouble, Cint),
                 p.h, nsteps, dt, dt_kick, 0.0, ckpt_every), p.h)
+    p.tape += 1
     return get_psi(p)
 end
 
 function ChainRulesCore.rrule(::typeof(ssfm), p::HIPPlan, ψ0::Array{ComplexF64,3}, θ::NTuple{5,Float64},
                               nsteps::Integer, dt::Real; dt_kick::Real=dt / 2, ckpt_every::Integer=20)
     ψN = ssfm(p, ψ0, θ, nsteps, dt; dt_kick, ckpt_every)
+    token = p.tape
     function ssfm_pullback(ΔψN)
+        # the plan holds one tape: a later ssfm on the same plan replaced it
+        p.tape == token || error("ssfm pullback: the plan was re-taped by a later ssfm call; " *
+                                 "use one HIPPlan per differentiated call")
         g = Array{ComplexF64,3}(unthunk(ΔψN))
         g0 = similar(g)
         gp = zeros(Float64, WDM_ADJ_NPARAM)
+        # the library also refuses (WDM_ERR_STATE) if the plan's models or ψ changed since
         check(ccall((:wdm_adjoint_backward, libwdm), Cint,
                     (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{Cvoid}, Ptr{Cdouble}, Cint, Cuint),
                     p.h, g, WDM_C128, g0, gp, WDM_ADJ_NPARAM, Cuint(0)), p.h)
@@ -315,5 +319,8 @@ function ChainRulesCore.rrule(::typeof(ssfm), p::HIPPlan, ψ0::Array{ComplexF64,
     end
     return ψN, ssfm_pullback
 end
+
+# methods of WaveDM's solver-step functions for a HIPDeviceConfig
+include("WaveDMHIPBackend.jl")
 
 end # module

@@ -15,6 +15,7 @@ time-step helpers are plain host logic, as in the reference.
   apply_kick_step                  src/solver/KDK.jl:37-77
   apply_drift_step                 src/solver/KDK.jl:88-101
   run_loop                         src/simulation_main.jl:536-551 (the fast path is Plan.step)
+  split_energy                     the energy the loop's split conserves (Q1 of the loop)
 """
 from __future__ import annotations
 
@@ -301,6 +302,22 @@ def apply_drift_step(spec: Plan, linear_phase: DriftOperator, border=None, gpu=T
     spec.set_border(border if border is not None and not np.all(np.asarray(border) == 1.0) else None)
     spec.drift(linear_phase.dt)
     return spec
+
+
+def split_energy(red, dt, dt_kick, split="reference"):
+    """The energy the loop's split conserves, from a ``Plan.reduce()`` dict.
+
+    One reference iteration is K(dt_kick) D(dt) (src/simulation_main.jl:536-547,
+    one kick of dt_kick = dt/2 when KDK_flag): to first order in dt it advances
+    the flow of H = K + (dt_kick/dt) (W + V_ext), i.e. Schroedinger-Poisson with
+    both the self-gravity and the external potential scaled by dt_kick/dt.
+    That H (kinetic + scaled self-gravity W = 1/2 sum rho Phi dV + scaled
+    sum rho V dV) is what a static potential keeps constant up to the
+    splitting error; K + W + V_ext is not (it moves at O(1) when
+    dt_kick != dt).  The Strang split K(dt/2) D(dt) K(dt/2) kicks for a whole
+    dt per step, so there H = K + W + V_ext."""
+    s = 1.0 if split == "strang" else float(dt_kick) / float(dt)
+    return red["kinetic"] + s * (red["w_self"] + red["v_ext"])
 
 
 def run_loop(plan: Plan, Nt, dt, *, KDK_flag=True, t0=0.0, snapshot_every=0, on_snapshot=None,

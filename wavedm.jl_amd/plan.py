@@ -129,6 +129,28 @@ class Plan:
         p = np.array([kappa], dtype=np.float64)
         self._chk(self.lib.wdm_set_potential_model(self._h, A.WDM_V_KAPPA, _dptr(p), 1))
 
+    def set_tidal(self, table, lo=None, hi=None, traj_t=None, traj_xyz=None):
+        """Milky Way tidal potential in every kick (src/physics/tidal_fields.jl:64-146,
+        applied; the reference computes and drops it).  ``table`` (tnx, tny, tnz)
+        on the uniform grid from ``lo`` to ``hi`` (3 each); ``traj_t`` (ntraj,)
+        kick times and ``traj_xyz`` (ntraj, 3) offsets: a kick at t uses the
+        last row with traj_t <= t.  ``table=None`` clears."""
+        if table is None:
+            self._chk(self.lib.wdm_set_tidal_model(self._h, None, 0, 0, 0, None, None, None, None, 0))
+            return
+        tab = np.asarray(table, dtype=np.float64)
+        if tab.ndim != 3:
+            raise ValueError("table must be 3-D")
+        lo = np.ascontiguousarray(lo, dtype=np.float64)
+        hi = np.ascontiguousarray(hi, dtype=np.float64)
+        tt = np.ascontiguousarray(np.atleast_1d(traj_t), dtype=np.float64)
+        tx = np.ascontiguousarray(np.asarray(traj_xyz, dtype=np.float64).reshape(-1, 3))
+        if lo.shape != (3,) or hi.shape != (3,) or tx.shape[0] != tt.size:
+            raise ValueError("lo and hi need 3 entries and traj_xyz one row of 3 per traj_t entry")
+        flat = np.ascontiguousarray(tab.ravel(order="F"))
+        self._chk(self.lib.wdm_set_tidal_model(self._h, _dptr(flat), *tab.shape, _dptr(lo), _dptr(hi), _dptr(tt),
+                                               _dptr(tx), int(tt.size)))
+
     def clear_potential_models(self):
         self._chk(self.lib.wdm_clear_potential_models(self._h))
 
