@@ -233,11 +233,13 @@ int wdm_ic_velocity(wdm_plan* plan, const double* rho, const double* vx, const d
      0..4  dL/dM, dL/da, dL/dcx, dL/dcy, dL/dcz of the Hernquist model
      5     dL/dsigma for dt -> sigma dt, dt_kick -> sigma dt_kick at sigma = 1
            (only with WDM_ADJ_DT_SCALE, else NaN; needs a time-independent V)
+     6..11 dL/dTxx, dL/dTyy, dL/dTzz, dL/dTxy, dL/dTxz, dL/dTyz of the
+           rotating quadrupole's T0 (the tidal tensor; 0 without the model)
    Segments of ckpt_every steps are recomputed during the backward sweep.
    Self-gravity, static V, Hernquist, quadrupole(t) and kappa|psi|^2 are
    differentiated; psi is left at psi_N. */
 #define WDM_ADJ_DT_SCALE 1u
-#define WDM_ADJ_NPARAM 6
+#define WDM_ADJ_NPARAM 12
 int wdm_adjoint_forward(wdm_plan* plan, int nsteps, double dt, double dt_kick, double t0, int ckpt_every);
 int wdm_adjoint_backward(wdm_plan* plan, const void* grad_psi, int dtype, void* grad_psi0, double* grad_params,
                          int nparams, unsigned flags);

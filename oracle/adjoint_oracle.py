@@ -7,9 +7,11 @@ compared against, never part of the product path.
 The forward map restates ssfm_oracle.run_ssfm with split=SPLIT_REFERENCE
 (src/simulation_main.jl:536-547, src/solver/KDK.jl:37-101,
 src/solver/gravity.jl:13-23), with the kick potential
-    Phi[psi] + V_H(x; M, a, c) + kappa |psi|^2
-where V_H = -G M / (|x - c| + a) is the Hernquist host and sigma scales dt and
-dt_kick.  The reference defines no adjoint (it imports Zygote,
+    Phi[psi] + V_H(x; M, a, c) + V_Q(x, t; T0) + kappa |psi|^2
+where V_H = -G M / (|x - c| + a) is the Hernquist host, V_Q = 1/2 u^T T0 u
+with u = R(t)^T (x - c_Q) the rotating tidal quadrupole (R(t) the rotation
+about the axis by 2 pi t / period; kick n at t0 + n dt), and sigma scales dt
+and dt_kick.  The reference defines no adjoint (it imports Zygote,
 src/WaveDM.jl:42, but has no rule for the step); this one is pinned by central
 finite differences of the forward map in tests/test_adjoint_oracle.py.
 
@@ -21,14 +23,22 @@ import numpy as np
 
 from . import ssfm_oracle as O
 
-PARAM_NAMES = ("M", "a", "cx", "cy", "cz", "sigma")
+PARAM_NAMES = ("M", "a", "cx", "cy", "cz", "sigma", "Txx", "Tyy", "Tzz", "Txy", "Txz", "Tyz")
+
+
+def rotation(axis, angle):
+    """Rotation by ``angle`` about ``axis`` (Rodrigues)."""
+    n = np.asarray(axis, dtype=np.float64)
+    n = n / np.linalg.norm(n)
+    K = np.array([[0.0, -n[2], n[1]], [n[2], 0.0, -n[0]], [-n[1], n[0], 0.0]])
+    return np.eye(3) + np.sin(angle) * K + (1.0 - np.cos(angle)) * (K @ K)
 
 
 class Problem:
     """Grid, coefficients and potential of one adjoint run."""
 
     def __init__(self, shape, Xmax, dt, dt_kick=None, *, hbar_over_m=1.0, four_pi_G=4.0 * np.pi,
-                 hernquist=None, kappa=0.0, border=None, self_gravity=True):
+                 hernquist=None, kappa=0.0, border=None, self_gravity=True, quadrupole=None, t0=0.0):
         self.shape = tuple(shape)
         self.Xmax = tuple(Xmax)
         nx, ny, nz = self.shape
@@ -41,6 +51,13 @@ class Problem:
         self.four_pi_G = float(four_pi_G)
         self.G = self.four_pi_G / (4.0 * np.pi)
         self.hernquist = None if hernquist is None else tuple(float(v) for v in hernquist)  # M, a, cx, cy, cz
+        # quadrupole: (T0 as xx, yy, zz, xy, xz, yz; axis; period; centre)
+        self.quadrupole = None
+        if quadrupole is not None:
+            T0, axis, period, cen = quadrupole
+            self.quadrupole = (tuple(float(v) for v in T0), tuple(float(v) for v in axis), float(period),
+                               tuple(float(v) for v in cen))
+        self.t0 = float(t0)
         self.kappa = float(kappa)
         self.border = None if border is None else np.asarray(border, dtype=np.float64)
         self.self_gravity = self_gravity
@@ -68,13 +85,32 @@ class Problem:
         """The linear Poisson operator of the forward: four_pi_G lap^-1 (s - <s>)."""
         return self.four_pi_G * np.real(O.parallel_poisson(self.delta, self.poisson_N, s - s.mean()))
 
-    def v_host(self, hq=None):
+    def v_host(self, hq=None, t=0.0, T0=None):
         hq = self.hernquist if hq is None else hq
-        if hq is None:
-            return np.zeros(self.shape)
-        M, a, cx, cy, cz = hq
-        r = np.sqrt((self.xxx - cx) ** 2 + (self.yyy - cy) ** 2 + (self.zzz - cz) ** 2)
-        return -self.G * M / (r + a)
+        v = np.zeros(self.shape)
+        if hq is not None:
+            M, a, cx, cy, cz = hq
+            r = np.sqrt((self.xxx - cx) ** 2 + (self.yyy - cy) ** 2 + (self.zzz - cz) ** 2)
+            v = v - self.G * M / (r + a)
+        if self.quadrupole is not None:
+            # V_Q = 1/2 u^T T0 u is linear in the six components of T0
+            T = self.quadrupole[0] if T0 is None else T0
+            for Tj, c in zip(T, self.dv_quad(t)):
+                v = v + Tj * c
+        return v
+
+    def quad_u(self, t):
+        """u = R(t)^T (x - c_Q) on the grid."""
+        _, axis, period, cen = self.quadrupole
+        ang = 2.0 * np.pi * t / period if period != 0.0 else 0.0
+        R = rotation(axis, ang)
+        d = (self.xxx - cen[0], self.yyy - cen[1], self.zzz - cen[2])
+        return [R[0, a] * d[0] + R[1, a] * d[1] + R[2, a] * d[2] for a in range(3)]
+
+    def dv_quad(self, t):
+        """dV_Q/d(Txx, Tyy, Tzz, Txy, Txz, Tyz) at time t (V_Q is linear in T0)."""
+        u = self.quad_u(t)
+        return [0.5 * u[0] ** 2, 0.5 * u[1] ** 2, 0.5 * u[2] ** 2, u[0] * u[1], u[0] * u[2], u[1] * u[2]]
 
     def dv_host(self):
         """dV_H/d(M, a, cx, cy, cz) on the grid."""
@@ -93,32 +129,37 @@ class Problem:
             out = self.border[:, None, None] * out
         return out
 
-    def step(self, psi, sigma=1.0, hq=None):
+    def kick_time(self, n, sigma=1.0):
+        return self.t0 + n * sigma * self.dt
+
+    def step(self, psi, sigma=1.0, hq=None, t=0.0, T0=None):
         tau = sigma * self.dt_kick / self.hbar_over_m
-        pot = self.phi(psi) + self.v_host(hq) + self.kappa * np.abs(psi) ** 2
+        pot = self.phi(psi) + self.v_host(hq, t, T0) + self.kappa * np.abs(psi) ** 2
         psip = np.exp(-1j * tau * pot) * psi
         return self.drift(psip, sigma)
 
-    def forward(self, psi0, nsteps, sigma=1.0, hq=None):
+    def forward(self, psi0, nsteps, sigma=1.0, hq=None, T0=None):
         psi = np.asarray(psi0, dtype=np.complex128)
-        for _ in range(nsteps):
-            psi = self.step(psi, sigma, hq)
+        for n in range(nsteps):
+            psi = self.step(psi, sigma, hq, self.kick_time(n, sigma), T0)
         return psi
 
     # -- adjoint ------------------------------------------------------------------
     def backward(self, psi0, nsteps, gN):
-        """(g0, dL/d[M, a, cx, cy, cz, sigma]) for the loss gradient gN at psi_N,
-        sigma = 1 and the problem's Hernquist parameters."""
+        """(g0, dL/d[M, a, cx, cy, cz, sigma, Txx, Tyy, Tzz, Txy, Txz, Tyz]) for
+        the loss gradient gN at psi_N, sigma = 1 and the problem's potential
+        parameters (zero for an absent model)."""
         states = [np.asarray(psi0, dtype=np.complex128)]
-        for _ in range(nsteps - 1):
-            states.append(self.step(states[-1]))
+        for n in range(nsteps - 1):
+            states.append(self.step(states[-1], t=self.kick_time(n)))
         tau = self.dt_kick / self.hbar_over_m
-        vh = self.v_host()
         dvh = self.dv_host() if self.hernquist is not None else None
-        grads = np.zeros(6)
+        grads = np.zeros(12)
         g = np.asarray(gN, dtype=np.complex128)
         for n in range(nsteps - 1, -1, -1):
             psi = states[n]
+            t = self.kick_time(n)
+            vh = self.v_host(t=t)
             pot = self.phi(psi) + vh + self.kappa * np.abs(psi) ** 2
             E = np.exp(-1j * tau * pot)
             psip = E * psi
@@ -132,6 +173,9 @@ class Problem:
             if dvh is not None:
                 for j in range(5):
                     grads[j] += np.sum(s * dvh[j])
+            if self.quadrupole is not None:
+                for j, d in enumerate(self.dv_quad(t)):
+                    grads[6 + j] += np.sum(s * d)
             u = (self.op_G(s) if self.self_gravity else 0.0) + self.kappa * s
             g = np.conj(E) * gp + 2.0 * u * psi
         return g, grads

@@ -84,3 +84,59 @@ def test_package_profile_loss_matches_oracle(wdm):
     L2, g2 = AO.profile_loss(psi0, idx, t, 10)
     assert abs(L1 - L2) <= 1e-14 * abs(L2)
     assert np.allclose(g1, g2, rtol=1e-14, atol=0)
+
+
+QUAD = ((30.0, -20.0, 10.0, 4.0, -2.0, 1.0), (0.2, 0.5, 1.0), 0.4, (0.1, 0.0, -0.2))
+
+
+def _setup_quad(t0=0.03):
+    pr, psi0, idx, rng = _setup(border=False, kappa=0.0)
+    pr = AO.Problem(pr.shape, pr.Xmax, dt=pr.dt, hernquist=pr.hernquist, four_pi_G=pr.four_pi_G,
+                    quadrupole=QUAD, t0=t0)
+    return pr, psi0, idx, rng
+
+
+def test_forward_with_rotating_quadrupole_equals_reference_step():
+    # kick n at t0 + n dt with V_Q(t) = 1/2 (x-c)^T R T0 R^T (x-c), as run_ssfm's host V
+    pr, psi0, _, _ = _setup_quad()
+    M, a, cx, cy, cz = pr.hernquist
+    T0v, axis, period, cen = QUAD
+    T0 = np.array([[T0v[0], T0v[3], T0v[4]], [T0v[3], T0v[1], T0v[5]], [T0v[4], T0v[5], T0v[2]]])
+
+    def V(xxx, yyy, zzz, ps, t):
+        R = AO.rotation(axis, 2 * np.pi * t / period)
+        T = R @ T0 @ R.T
+        d = [xxx - cen[0], yyy - cen[1], zzz - cen[2]]
+        vq = 0.5 * sum(T[i, j] * d[i] * d[j] for i in range(3) for j in range(3))
+        return vq - pr.G * M / (np.sqrt((xxx - cx) ** 2 + (yyy - cy) ** 2 + (zzz - cz) ** 2) + a)
+
+    ref = O.run_ssfm(psi0, pr.Xmax, 4, pr.dt, hbar_over_m=1.0, four_pi_G=pr.four_pi_G, V=V, t0=pr.t0)
+    got = pr.forward(psi0, 4)
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-13
+    # the rotation matters over these kicks
+    assert np.linalg.norm(AO.Problem(pr.shape, pr.Xmax, dt=pr.dt, hernquist=pr.hernquist,
+                                     four_pi_G=pr.four_pi_G, quadrupole=(T0v, axis, 0.0, cen),
+                                     t0=pr.t0).forward(psi0, 4) - ref) > 1e-4 * np.linalg.norm(ref)
+
+
+def test_quadrupole_gradients_match_central_differences():
+    pr, psi0, idx, rng = _setup_quad()
+    nsteps = 5
+    target = _target(pr, psi0, idx, nsteps)
+    L, gN = AO.profile_loss(pr.forward(psi0, nsteps), idx, target, 6)
+    g0, gp = pr.backward(psi0, nsteps, gN)
+    T0 = np.array(QUAD[0])
+    for j in range(6):
+        e = np.zeros(6)
+        e[j] = 1e-5 * max(1.0, abs(T0[j]))
+        lp = AO.profile_loss(pr.forward(psi0, nsteps, T0=tuple(T0 + e)), idx, target, 6)[0]
+        lm = AO.profile_loss(pr.forward(psi0, nsteps, T0=tuple(T0 - e)), idx, target, 6)[0]
+        fd = (lp - lm) / (2 * e[j])
+        assert abs(fd - gp[6 + j]) < 1e-5 * abs(fd), (AO.PARAM_NAMES[6 + j], fd, gp[6 + j])
+    # the Hernquist gradients are unchanged by the extra model
+    hq = np.array(pr.hernquist)
+    e = np.zeros(5)
+    e[0] = 1e-6 * hq[0]
+    lp = AO.profile_loss(pr.forward(psi0, nsteps, hq=tuple(hq + e)), idx, target, 6)[0]
+    lm = AO.profile_loss(pr.forward(psi0, nsteps, hq=tuple(hq - e)), idx, target, 6)[0]
+    assert abs((lp - lm) / (2 * e[0]) - gp[0]) < 1e-5 * abs(gp[0])

@@ -60,6 +60,37 @@ def test_adjoint_matches_oracle(wdm, nativelib, shape, border, kappa):
         assert np.all(perr < tol * 10 if prec == "f32" else perr < tol), (prec, gp, p_ref)
 
 
+QUAD = ((30.0, -20.0, 10.0, 4.0, -2.0, 1.0), (0.2, 0.5, 1.0), 0.4, (0.1, 0.0, -0.2))
+
+
+@pytest.mark.parametrize("shape", [(16, 12, 8), (12, 8, 16)], ids=["16x12x8", "12x8x16"])
+def test_quadrupole_gradients_match_oracle(wdm, nativelib, shape):
+    # the rotating tidal quadrupole's T0 (grad_params 6..11) with the Hernquist
+    # host, kicks at t0 + n dt; dt scale off (the potential is time-dependent)
+    t0 = 0.03
+    pr = AO.Problem(shape, X3, dt=0.05, hernquist=HQ, four_pi_G=FOUR_PI_G, quadrupole=QUAD, t0=t0)
+    psi0 = _psi0(shape, seed=7)
+    nsteps = 6
+    loss = wdm.adjoint.ProfileLoss(shape, X3, nbins=8)
+    target = loss.profile(pr.forward(psi0, nsteps)) * 1.2
+    L_ref, gN = loss(pr.forward(psi0, nsteps), target)
+    g_ref, p_ref = pr.backward(psi0, nsteps, gN)
+    sel = [0, 1, 2, 3, 4, 6, 7, 8, 9, 10, 11]
+    assert np.all(np.abs(p_ref[6:]) > 0)
+    for prec, tol in (("f64", 1e-9), ("f32", 2e-3)):
+        with _plan(wdm, nativelib, pr, prec) as p:
+            T0v, axis, period, cen = QUAD
+            p.set_quadrupole(T0v, axis, period, cen)
+            L, g0, gp = wdm.adjoint.value_and_grad(p, psi0, nsteps, pr.dt, loss, target, t0=t0, ckpt_every=4,
+                                                   dt_scale=False)
+        assert abs(L - L_ref) <= tol * abs(L_ref), (prec, L, L_ref)
+        err = np.linalg.norm(g0 - g_ref) / np.linalg.norm(g_ref)
+        assert err < tol, (prec, err)
+        perr = np.abs(gp[sel] - p_ref[sel]) / np.abs(p_ref[sel])
+        assert np.all(perr < (tol * 10 if prec == "f32" else tol)), (prec, gp, p_ref)
+        assert np.isnan(gp[5])
+
+
 def test_checkpoint_interval_does_not_change_the_gradient(wdm, nativelib):
     pr = _problem((16, 12, 8), True, 0.7)
     psi0 = _psi0(pr.shape, seed=5)

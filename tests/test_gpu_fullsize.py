@@ -218,11 +218,12 @@ def test_c4_fp32_gradient_matches_fp64(wdm, nativelib):
     dd32 = float(np.sum(np.real(np.conj(g32) * d)))
     full = float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64))
     print(f"  L {L64:.8e} / {L32:.8e}; psi0 direction {dd64:.8e} / {dd32:.8e}; full rel L2 {full:.3e}", flush=True)
-    for j, name in enumerate(wdm.adjoint.PARAM_NAMES):
+    names = wdm.adjoint.PARAM_NAMES[:6]  # Hernquist and dt scale (no quadrupole in this workload)
+    for j, name in enumerate(names):
         print(f"  {name}: {p64[j]:.8e} / {p32[j]:.8e} (rel {abs(p32[j] - p64[j]) / abs(p64[j]):.2e})", flush=True)
     assert abs(L32 - L64) <= 1e-4 * abs(L64)
     assert abs(dd32 - dd64) <= GRAD_F32_BAR * abs(dd64), (dd32, dd64)
-    for j, name in enumerate(wdm.adjoint.PARAM_NAMES):
+    for j, name in enumerate(names):
         assert abs(p32[j] - p64[j]) <= GRAD_F32_BAR * abs(p64[j]), (name, p32[j], p64[j])
 
 

@@ -43,7 +43,7 @@ WDM_RED_PSIMAX = 4
 WDM_RED_PHIMAX = 5
 WDM_RED_COUNT = 6
 WDM_ADJ_DT_SCALE = 1
-WDM_ADJ_NPARAM = 6
+WDM_ADJ_NPARAM = 12
 WDM_DIAG_MASS = 0
 WDM_DIAG_RHOMAX = 1
 WDM_DIAG_PEAK_IX = 2

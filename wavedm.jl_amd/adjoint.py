@@ -11,7 +11,7 @@ from __future__ import annotations
 
 import numpy as np
 
-PARAM_NAMES = ("M", "a", "cx", "cy", "cz", "sigma")
+PARAM_NAMES = ("M", "a", "cx", "cy", "cz", "sigma", "Txx", "Tyy", "Tzz", "Txy", "Txz", "Tyz")
 
 
 def _coords(n, X):
@@ -63,7 +63,9 @@ class ProfileLoss:
 def value_and_grad(plan, psi0, nsteps, dt, loss, target, dt_kick=None, t0=0.0, ckpt_every=20, dt_scale=True):
     """Run ``nsteps`` reference-split steps from ``psi0`` on the device with
     checkpoints, evaluate ``loss(psi_N, target)`` and return
-    (L, dL/dpsi0 packed as dL/dRe + i dL/dIm, dL/d[M, a, cx, cy, cz, sigma]).
+    (L, dL/dpsi0 packed as dL/dRe + i dL/dIm, the WDM_ADJ_NPARAM parameter
+    gradients named by PARAM_NAMES: Hernquist M, a, centre; the dt scale
+    sigma (NaN unless ``dt_scale``); the quadrupole's T0 components).
     This is the call sequence of the rrule's forward and pullback."""
     plan.set_psi(psi0)
     plan.adjoint_forward(nsteps, dt, dt_kick, t0, ckpt_every)

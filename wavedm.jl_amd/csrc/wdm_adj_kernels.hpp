@@ -19,7 +19,9 @@
 
 namespace wdm {
 
-constexpr int kAdjParams = 6;  // dM, da, dcx, dcy, dcz of the Hernquist model; sum s (Phi + V)
+// parameter sums: dM, da, dcx, dcy, dcz of the Hernquist model; sum s (Phi + V)
+// (dt scale); dTxx, dTyy, dTzz, dTxy, dTxz, dTyz of the quadrupole's T0
+constexpr int kAdjParams = 12;
 
 struct AdjXArgs {
     double tau;                  // (m/hbar) dt_kick
@@ -30,6 +32,8 @@ struct AdjXArgs {
     const double* phi;           // taped real Phi_n (FP64) or null (no self-gravity)
     const double* border;        // length-Nx absorber or null (k_adj_xg only)
     VModel vm;                   // V at the step's kick time (includes the kappa term)
+    double qR[9];                // quadrupole rotation R(t) at the kick time, row-major:
+                                 // V = 1/2 u^T T0 u with u = R^T (x - c)
 };
 
 // E = exp(-i theta) as (cos r, sin r, quadrant q) of theta = q pi/2 + r
@@ -67,17 +71,19 @@ WDM_FI double adj_potential(const AdjXArgs& a, const LineV& lv, int x, unsigned 
 
 // In: g rows transformed along x.  Out: g' in real space (in place), the
 // half spectrum of s along x into shat (when given) and kAdjParams partial
-// sums per workgroup into partials[kAdjParams * blockIdx.x + j].
-template <class T, class P, class F, int NW>
+// sums per workgroup into partials[kAdjParams * blockIdx.x + j] (the
+// quadrupole's six only when QG, zeros otherwise).
+template <class T, class P, class F, int NW, bool QG>
 WDM_KERNEL(NW * 64)
 void k_adj_xs(cx<T>* __restrict__ g, const cx<T>* __restrict__ psi_n, cx<T>* __restrict__ shat, int nlines,
               int hpitch, const typename TwEntry<T>::type* __restrict__ tw, AdjXArgs a,
               double* __restrict__ partials) {
     constexpr int N1 = F::N1, N2 = F::N2, M = F::M, N = F::N;
+    constexpr int NP = QG ? kAdjParams : 6;
     WDM_SHARED cx<T> xbuf[NW][F::XBUF];
     WDM_SHARED typename TwEntry<T>::type tw_s[F::N];
-    WDM_SHARED double red[NW][kAdjParams * kWave];
-    WDM_SHARED double wred[NW][kAdjParams];
+    WDM_SHARED double red[NW][NP * kWave];
+    WDM_SHARED double wred[NW][NP];
     tw = stage_table<F::N>(tw_s, tw);
     const LaneMap m = lane_map<F, NW>();
     const int l = m.w * F::LPW + m.lw;
@@ -103,7 +109,17 @@ void k_adj_xs(cx<T>* __restrict__ g, const cx<T>* __restrict__ psi_n, cx<T>* __r
     const T hy = T(a.ys[y] - a.vm.hq_c[1]), hz = T(a.zs[z] - a.vm.hq_c[2]);
     const T hr2 = hy * hy + hz * hz;
     const T tau = T(a.tau), G = T(a.vm.hq_G), GM = T(a.vm.hq_GM), ha = T(a.vm.hq_a);
-    double acc[kAdjParams] = {0, 0, 0, 0, 0, 0};
+    double acc[NP] = {};
+    // quadrupole: u = R^T (x - c) = R[0][.] dx + (the line's R[1][.] dy + R[2][.] dz)
+    T qr0[3] = {}, qw[3] = {};
+    if constexpr (QG) {
+        const double dy = a.ys[y] - a.vm.q_c[1], dz = a.zs[z] - a.vm.q_c[2];
+        WDM_UNROLL
+        for (int b = 0; b < 3; ++b) {
+            qr0[b] = T(a.qR[b]);
+            qw[b] = T(a.qR[3 + b] * dy + a.qR[6 + b] * dz);
+        }
+    }
     cx<T> sr[N1];
     WDM_UNROLL
     for (int kq = 0; kq < N1; ++kq) {
@@ -130,12 +146,24 @@ void k_adj_xs(cx<T>* __restrict__ g, const cx<T>* __restrict__ psi_n, cx<T>* __r
             }
         }
         acc[5] += double(sv) * ph;
+        if constexpr (QG) {
+            // dV/dT0: 1/2 u_a^2 on the diagonal, u_a u_b off it
+            const T dxq = T(a.xs[x] - a.vm.q_c[0]);
+            const T u0 = fmaT(qr0[0], dxq, qw[0]), u1 = fmaT(qr0[1], dxq, qw[1]), u2 = fmaT(qr0[2], dxq, qw[2]);
+            const T hs = T(0.5) * sv;
+            acc[6] += double(hs * u0 * u0);
+            acc[7] += double(hs * u1 * u1);
+            acc[8] += double(hs * u2 * u2);
+            acc[9] += double(sv * u0 * u1);
+            acc[10] += double(sv * u0 * u2);
+            acc[11] += double(sv * u1 * u2);
+        }
     }
     // workgroup sums in a fixed order: lanes, then waves
     WDM_UNROLL
-    for (int j = 0; j < kAdjParams; ++j) red[m.wv][j * kWave + m.lane] = acc[j];
+    for (int j = 0; j < NP; ++j) red[m.wv][j * kWave + m.lane] = acc[j];
     WDM_WAVE_SYNC();
-    if (m.lane < kAdjParams) {
+    if (m.lane < NP) {
         double s = 0.0;
         for (int k = 0; k < kWave; ++k) s += red[m.wv][m.lane * kWave + k];
         wred[m.wv][m.lane] = s;
@@ -143,7 +171,8 @@ void k_adj_xs(cx<T>* __restrict__ g, const cx<T>* __restrict__ psi_n, cx<T>* __r
     __syncthreads();
     if (int(threadIdx.x) < kAdjParams) {
         double s = 0.0;
-        for (int w = 0; w < NW; ++w) s += wred[w][threadIdx.x];
+        if (int(threadIdx.x) < NP)
+            for (int w = 0; w < NW; ++w) s += wred[w][threadIdx.x];
         partials[kAdjParams * blockIdx.x + threadIdx.x] = s;
     }
     if (shat) {

@@ -351,6 +351,7 @@ void adj_backward(wdm_plan* p, const void* grad, int dtype, void* grad0, double*
             xa.phi = fn;
             xa.border = nullptr;
             xa.vm = make_vmodel(p, A.t0 + double(step) * A.dt);
+            quad_rotation(p, A.t0 + double(step) * A.dt, xa.qR);
             adj_conj_drift_yz<T>(p, g, tspec ? tspec + n * size_t(i) : nullptr, part_spec + size_t(step) * nb);
             cx<T>* rh = p->self_gravity ? p->rhohat.as<cx<T>>() : nullptr;
             ox->adj_xs(g, pn, rh, nlines, p->hpitch, p->ax[0].tw.as<TwE>(), xa,
@@ -387,6 +388,7 @@ void adj_backward(wdm_plan* p, const void* grad, int dtype, void* grad0, double*
     if (gparams) {
         for (int j = 0; j < nparams && j < kAdjNV; ++j) gparams[j] = h[j];
         if (nparams > 5 && !want_dt) gparams[5] = NAN;
+        for (int j = kAdjNV; j < nparams; ++j) gparams[j] = NAN;
     }
 }
 

@@ -611,13 +611,38 @@ constexpr int col_min_waves() {
     return 1;
 }
 
+// WDM_COL_EARLY_LOADS: the column passes issue their data loads before
+// staging the tables, so the table fetch and the barrier after it overlap the
+// data's memory latency instead of preceding it.
+#ifndef WDM_COL_EARLY_LOADS
+#define WDM_COL_EARLY_LOADS 0
+#endif
+
 template <class T, class P, class F, int MODE, int KF, int NW>
 WDM_KERNEL_W(NW * 64, (col_min_waves<T, MODE>()))
 void k_col(cx<T>* __restrict__ data, int ncols, int nother, unsigned sline, unsigned sother,
            const typename TwEntry<T>::type* __restrict__ tw, KTabs kt) {
     constexpr int N1 = F::N1, N2 = F::N2, M = F::M;
+    constexpr bool EARLY = WDM_COL_EARLY_LOADS && MODE != COL_MI;
     WDM_SHARED cx<T> xbuf[NW][F::XBUF];
     WDM_SHARED typename TwEntry<T>::type tw_s[F::N];
+    const LaneMap m = lane_map<F, NW>();
+    const int ntiles = (ncols + F::LPW - 1) / F::LPW;
+    const int wl = int(xcd_block()) * NW + m.wv;
+    const int o = wl / ntiles, tile = wl % ntiles;
+    const int c = tile * F::LPW + m.lw;
+    const bool ok = (o < nother) && (c < ncols);
+    const int oc = ok ? o : 0, cc = ok ? c : 0;
+    // 32-bit element offsets from the uniform base pointer: one VGPR per
+    // address (scalar-base addressing) and cheap to recompute for the stores
+    const unsigned base = unsigned(oc) * sother + unsigned(cc) + unsigned(m.j) * sline;
+    const unsigned step = unsigned(N2) * sline;
+    const cx<T> zero = mk(T(0), T(0));
+    cx<T> v[N1];
+    if constexpr (EARLY) {
+        WDM_UNROLL
+        for (int n1 = 0; n1 < N1; ++n1) v[n1] = WDM_LDG(ok, data, base + unsigned(n1) * step);
+    }
     // k-space factor table of the line axis, staged with the twiddles
     typedef typename LineEntry<KF, T>::type LE;
     constexpr int NL = (KF == KF_DRIFT || KF == KF_POISSON) ? F::N : 1;
@@ -628,21 +653,8 @@ void k_col(cx<T>* __restrict__ data, int ncols, int nother, unsigned sline, unsi
         kt.line = line_s;
     }
     tw = stage_table<F::N>(tw_s, tw);
-    const LaneMap m = lane_map<F, NW>();
-    const int ntiles = (ncols + F::LPW - 1) / F::LPW;
-    const int wl = int(xcd_block()) * NW + m.wv;
-    const int o = wl / ntiles, tile = wl % ntiles;
-    const int c = tile * F::LPW + m.lw;
-    const bool ok = (o < nother) && (c < ncols);
-    const int oc = ok ? o : 0, cc = ok ? c : 0;
     const KfLane<KF, T> kl = kf_lane<KF, T>(kt, cc, oc);
-    // 32-bit element offsets from the uniform base pointer: one VGPR per
-    // address (scalar-base addressing) and cheap to recompute for the stores
-    const unsigned base = unsigned(oc) * sother + unsigned(cc) + unsigned(m.j) * sline;
-    const unsigned step = unsigned(N2) * sline;
     cx<T>* xb = &xbuf[m.wv][m.lw * F::XS];
-    const cx<T> zero = mk(T(0), T(0));
-    cx<T> v[N1];
     cx<T> X[M][N2];
     if constexpr (MODE == COL_MI) {
         WDM_UNROLL
@@ -650,7 +662,7 @@ void k_col(cx<T>* __restrict__ data, int ncols, int nother, unsigned sline, unsi
             const int k = N2 * n1 + m.j;
             v[n1] = apply_kf<KF, P, T>(WDM_LDG(ok, data, base + unsigned(n1) * step), kl, kt, k);
         }
-    } else {
+    } else if constexpr (!EARLY) {
         WDM_UNROLL
         for (int n1 = 0; n1 < N1; ++n1) v[n1] = WDM_LDG(ok, data, base + unsigned(n1) * step);
     }

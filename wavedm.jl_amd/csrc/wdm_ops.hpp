@@ -239,8 +239,12 @@ struct Launch {
     }
     static void adj_xs(cx<T>* g, const cx<T>* pn, cx<T>* sh, int nlines, int hp, const TwE* tw, const AdjXArgs& a,
                        double* partials, hipStream_t s) {
-        WDM_LAUNCH((k_adj_xs<T, PsiP, F, NW>), dim3(x_blocks<NW, F>(nlines)), dim3(NW * 64), s, g, pn, sh, nlines, hp,
-                   tw, a, partials);
+        if (a.vm.flags & V_QUADRUPOLE)
+            WDM_LAUNCH((k_adj_xs<T, PsiP, F, NW, true>), dim3(x_blocks<NW, F>(nlines)), dim3(NW * 64), s, g, pn, sh,
+                       nlines, hp, tw, a, partials);
+        else
+            WDM_LAUNCH((k_adj_xs<T, PsiP, F, NW, false>), dim3(x_blocks<NW, F>(nlines)), dim3(NW * 64), s, g, pn, sh,
+                       nlines, hp, tw, a, partials);
     }
     static void adj_xg(bool fwd, cx<T>* g, const cx<T>* pn, const cx<T>* uh, int nlines, int hp, const TwE* tw,
                        const AdjXArgs& a, hipStream_t s) {

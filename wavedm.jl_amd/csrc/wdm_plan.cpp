@@ -363,6 +363,18 @@ void rotation_matrix(const double* n_in, double ang, double R[3][3]) {
     R[2][2] = c + n[2] * n[2] * C;
 }
 
+// R(t) of the rotating quadrupole (row-major; identity without the model)
+void quad_rotation(const wdm_plan* p, double t, double* R9) {
+    double R[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    if (p->vflags & V_QUADRUPOLE) {
+        const double period = p->quad[9];
+        const double ang = period != 0.0 ? 2.0 * M_PI * t / period : 0.0;
+        rotation_matrix(p->quad + 6, ang, R);
+    }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) R9[3 * i + j] = R[i][j];
+}
+
 VModel make_vmodel(const wdm_plan* p, double t) {
     VModel vm;
     std::memset(&vm, 0, sizeof(vm));

@@ -19,7 +19,7 @@ module WaveDMHIP
 using ChainRulesCore
 
 export HIPPlan, ssfm, set_psi!, get_psi, set_border!, set_static_potential!, set_hernquist!,
-       set_quadrupole!, set_kappa!, poisson, potential, kick!, drift!, step!, reduce_diagnostics,
+       set_quadrupole!, set_kappa!, set_tidal_model!, poisson, potential, kick!, drift!, step!, reduce_diagnostics,
        split_energy, cfl_limits, step_properties, accelerations, density_accumulate!, density_average,
        density_reset!, set_mass_ledger!, set_ic_velocity!, baryon_mesh_potential
 
@@ -38,7 +38,7 @@ const WDM_V_KAPPA = Cint(8)
 const WDM_RED_COUNT = 6
 const WDM_DIAG_COUNT = 22
 const WDM_ADJ_DT_SCALE = Cuint(1)
-const WDM_ADJ_NPARAM = 6
+const WDM_ADJ_NPARAM = 12
 
 struct WaveDMHIPError <: Exception
     code::Int
@@ -134,6 +134,34 @@ set_quadrupole!(p::HIPPlan; T0::AbstractMatrix, axis=(0.0, 0.0, 1.0), period=1.0
     _model!(p, WDM_V_QUADRUPOLE, Float64[T0[1, 1], T0[2, 2], T0[3, 3], T0[1, 2], T0[1, 3], T0[2, 3],
                                          axis..., period, center...])
 set_kappa!(p::HIPPlan, κ::Real) = _model!(p, WDM_V_KAPPA, Float64[κ])
+
+"""
+    set_tidal_model!(p, MW_Phi, lo, hi, traj_t, traj_xyz)
+
+The MW tidal potential in every kick (src/physics/tidal_fields.jl:64-146, applied: the
+reference's `Φ_all += pot_tidal` never reaches its kick).  `MW_Phi` on the uniform grid
+from `lo` to `hi`; `traj_t` the kick times of the `traj_xyz` rows (an n×3 matrix, the
+df_traj positions over uL): a kick at t uses the last row with traj_t ≤ t.
+`MW_Phi = nothing` removes it.
+"""
+function set_tidal_model!(p::HIPPlan, MW_Phi::Union{Nothing,Array{<:Real,3}}, lo=(0.0, 0.0, 0.0),
+                          hi=(1.0, 1.0, 1.0), traj_t::AbstractVector{<:Real}=Float64[],
+                          traj_xyz::AbstractMatrix{<:Real}=zeros(0, 3))
+    if MW_Phi === nothing
+        check(ccall((:wdm_set_tidal_model, libwdm), Cint,
+                    (Ptr{Cvoid}, Ptr{Cdouble}, Cint, Cint, Cint, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble},
+                     Ptr{Cdouble}, Cint), p.h, C_NULL, 0, 0, 0, C_NULL, C_NULL, C_NULL, C_NULL, 0), p.h)
+        return p
+    end
+    tab = Array{Float64,3}(MW_Phi)
+    rows = permutedims(Matrix{Float64}(traj_xyz))  # x, y, z of each row contiguous
+    check(ccall((:wdm_set_tidal_model, libwdm), Cint,
+                (Ptr{Cvoid}, Ptr{Cdouble}, Cint, Cint, Cint, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble},
+                 Ptr{Cdouble}, Cint),
+                p.h, tab, size(tab)..., Float64[lo...], Float64[hi...], Vector{Float64}(traj_t), rows,
+                length(traj_t)), p.h)
+    return p
+end
 
 """
     poisson(p::HIPPlan, ρ::Array{Float64,3}; scale=1.0) -> Φ

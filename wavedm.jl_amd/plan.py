@@ -348,8 +348,9 @@ class Plan:
 
     def adjoint_backward(self, grad_psi, dt_scale=False):
         """Given dL/dRe(psi_N) + i dL/dIm(psi_N), return (grad_psi0, grad_params)
-        with grad_params = [dL/dM, dL/da, dL/dcx, dL/dcy, dL/dcz, dL/dsigma]
-        (dL/dsigma is NaN unless ``dt_scale``)."""
+        with grad_params = [dL/dM, dL/da, dL/dcx, dL/dcy, dL/dcz, dL/dsigma,
+        dL/dTxx, dL/dTyy, dL/dTzz, dL/dTxy, dL/dTxz, dL/dTyz] (dL/dsigma is NaN
+        unless ``dt_scale``; a model that is not set has zero gradients)."""
         g = np.asarray(grad_psi)
         if g.shape != self.shape:
             raise ValueError(f"grad_psi must have shape {self.shape}")
