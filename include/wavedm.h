@@ -315,6 +315,24 @@ int wdm_density_restore(wdm_plan* plan, const double* host_sum, long long count)
 int wdm_ic_acceleration(wdm_plan* plan, const double* rho_halo, double* phi, double* ax, double* ay, double* az,
                         double* a_cdm);
 
+/* Allocation-free Poisson solve.  No counterpart in the reference: it is for
+   callers that cannot allocate device memory (code under HIP graph capture,
+   tight serving loops, harnesses that forbid hipMalloc).  The variant of
+   wdm_poisson(plan, rho, phi, scale) on caller-owned device buffers, with no
+   plan: phi_dev = scale * lap^-1 (rho_dev - <rho_dev>) on the periodic
+   lattice of period 2*{x,y,z}max (k = 0 -> 0), both arrays nx*ny*nz FP64,
+   x fastest; the arithmetic in `precision`, with the same kernels and tables
+   as a fresh plan's wdm_poisson (equal results).  scale must be nonzero.
+   Its work memory (half spectrum + tables) is a file-scope device array of
+   WDM_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; a grid
+   that needs more is refused (WDM_ERR_UNSUPPORTED).  The call allocates
+   nothing, is asynchronous on `stream` (a hipStream_t, NULL = the default
+   stream) and leaves the result ready when the stream is; calls that overlap
+   on one GPU share the array and race.  Errors: wdm_last_error(NULL). */
+#define WDM_NOALLOC_SCRATCH_BYTES (64u << 20)
+int wdm_poisson_noalloc(int nx, int ny, int nz, double xmax, double ymax, double zmax, int precision,
+                        const double* rho_dev, double* phi_dev, double scale, void* stream);
+
 /* Batched ensemble of independent plans (uneven grids; no counterpart in the
    reference, which steps one box at a time).  The plans stay owned by the
    caller and must outlive the ensemble.  wdm_ensemble_step runs nsteps

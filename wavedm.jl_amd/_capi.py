@@ -57,6 +57,7 @@ WDM_DIAG_VIRIAL = 21
 WDM_DIAG_COUNT = 22
 WDM_DIAG_BINS = 4096
 WDM_MAX_VARIANTS = 3
+WDM_NOALLOC_SCRATCH_BYTES = 64 << 20
 
 # every symbol include/wavedm.h declares: name -> (restype, argtypes)
 _P = C.c_void_p
@@ -95,6 +96,7 @@ SIGNATURES = {
     "wdm_set_mass_ledger": (_I, [_P, _I]),
     "wdm_get_mass_ledger": (_I, [_P, C.POINTER(_I), _DP]),
     "wdm_set_mass_reference": (_I, [_P, _D]),
+    "wdm_poisson_noalloc": (_I, [_I, _I, _I, _D, _D, _D, _I, _P, _P, _D, _P]),
     "wdm_ic_acceleration": (_I, [_P, _DP, _DP, _DP, _DP, _DP, _DP]),
     "wdm_ic_velocity": (_I, [_P, _DP, _DP, _DP, _DP, _I]),
     "wdm_step_timed_mask": (_I, [_P, _I, C.c_double, C.c_double, C.c_double, C.c_uint, _DP, _I, _DP]),

@@ -1541,4 +1541,5 @@ int wdm_synchronize(wdm_plan* p) {
 #include "wdm_ensemble.inl"
 #include "wdm_diag.inl"
 #include "wdm_ic.inl"
+#include "wdm_noalloc.inl"
 #include "wdm_tidal.inl"
