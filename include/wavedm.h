@@ -337,10 +337,14 @@ int wdm_poisson_noalloc(int nx, int ny, int nz, double xmax, double ymax, double
    reference, which steps one box at a time).  The plans stay owned by the
    caller and must outlive the ensemble.  wdm_ensemble_step runs nsteps
    reference-split steps of every member (member i with dt[i], dt_kick[i],
-   kick k at t0 + k dt[i]).  use_graph = 1: one step of all members is
-   captured once into a HIP graph (each member on its own stream) and
-   launched once per step; needs time-independent potentials.  use_graph = 0:
-   the same launches issued eagerly.  wdm_ensemble_autotune tunes each
+   kick k at t0 + k dt[i]).  Members of one shape, precision, pass table,
+   gravity, absorber and potential kind (without the mass projection or a
+   time-dependent potential) form a batch: each pass is one launch over all
+   of them; the others step on their own streams.  Every member's result is
+   the same, bit for bit, as stepping it alone.  use_graph = 1: one step of
+   all members is captured once into a HIP graph and launched once per step;
+   needs time-independent potentials.  use_graph = 0: the same launches issued
+   eagerly.  wdm_ensemble_autotune tunes each
    distinct (shape, precision) once and gives the choice to every member of
    that shape.  Errors: wdm_last_error(NULL). */
 typedef struct wdm_ensemble wdm_ensemble;
@@ -349,6 +353,9 @@ int wdm_ensemble_destroy(wdm_ensemble* ens);
 int wdm_ensemble_step(wdm_ensemble* ens, int nsteps, const double* dt, const double* dt_kick, double t0,
                       int use_graph);
 int wdm_ensemble_autotune(wdm_ensemble* ens, int trial_steps, const double* dt);
+/* How the last wdm_ensemble_step ran: *nbatches groups of >= 2 members that
+   share one launch per pass, *nsolo members stepped on their own streams. */
+int wdm_ensemble_layout(wdm_ensemble* ens, int* nbatches, int* nsolo);
 
 #ifdef __cplusplus
 }

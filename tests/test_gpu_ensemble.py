@@ -54,6 +54,51 @@ def test_ensemble_equals_members_stepped_alone(wdm, nativelib, graph):
             p.close()
 
 
+# members that share a shape, precision and potential kind step in batches
+# (one launch per pass for all of them); the rest alone
+BATCHED = [((32, 16, 8), "f32", None, 0.010), ((32, 16, 8), "f32", None, 0.020), ((32, 16, 8), "f32", "hernquist", 0.01),
+           ((16, 12, 8), "f64", "hernquist", 0.012), ((16, 12, 8), "f64", "hernquist2", 0.008),
+           ((24, 8, 16), "f32", "border", 0.015), ((24, 8, 16), "f32", "border", 0.01),
+           ((16, 16, 16), "f32", "nograv", 0.01), ((16, 16, 16), "f32", "nograv", 0.02),
+           ((48, 1, 16), "f64", None, 0.01), ((48, 1, 16), "f64", None, 0.02), ((32, 16, 8), "f32", "ledger", 0.01)]
+
+
+def _make_b(wdm, lib, shape, prec, extra, seed):
+    if extra == "nograv":
+        p = wdm.Plan(shape, X3, prec, self_gravity=False, lib=lib)
+        p.set_psi(blob(shape, seed))
+        return p
+    p = _make(wdm, lib, shape, prec, extra if extra != "hernquist2" else None, seed)
+    if extra == "hernquist2":
+        p.set_hernquist(30.0, 0.7, (-0.4, 0.2, 0.1))
+    if extra == "ledger":
+        p.set_mass_ledger(True)
+    return p
+
+
+@pytest.mark.parametrize("graph", [True, False], ids=["graph", "eager"])
+def test_batched_members_equal_members_stepped_alone(wdm, nativelib, graph):
+    ens_plans = [_make_b(wdm, nativelib, s, pr, ex, i) for i, (s, pr, ex, _) in enumerate(BATCHED)]
+    ref_plans = [_make_b(wdm, nativelib, s, pr, ex, i) for i, (s, pr, ex, _) in enumerate(BATCHED)]
+    dts = [m[3] for m in BATCHED]
+    try:
+        with wdm.Ensemble(ens_plans) as e:
+            e.step(3, dts, graph=graph)
+            e.step(2, dts, graph=graph)
+            # five batches: the two plain (32,16,8), the Hernquist (16,12,8) pair,
+            # the absorber pair, the pair without gravity, the (48,1,16) pair
+            assert e.layout() == (5, 2)
+        for p, dt in zip(ref_plans, dts):
+            for _ in range(5):
+                p.step(1, dt)
+        for a, b, m in zip(ens_plans, ref_plans, BATCHED):
+            dt = np.complex64 if a.precision == "f32" else np.complex128
+            assert np.array_equal(a.get_psi(dt), b.get_psi(dt)), m
+    finally:
+        for p in ens_plans + ref_plans:
+            p.close()
+
+
 def test_graph_recaptures_on_changes(wdm, nativelib):
     shape = (32, 16, 8)
     a, b = _make(wdm, nativelib, shape, "f32", None, 1), _make(wdm, nativelib, shape, "f32", None, 1)

@@ -112,6 +112,7 @@ SIGNATURES = {
     "wdm_ensemble_destroy": (_I, [_P]),
     "wdm_ensemble_step": (_I, [_P, _I, _DP, _DP, _D, _I]),
     "wdm_ensemble_autotune": (_I, [_P, _I, _DP]),
+    "wdm_ensemble_layout": (_I, [_P, C.POINTER(_I), C.POINTER(_I)]),
 }
 PASS_NAMES = ["x_kick_fwd", "y_fwd", "z_drift", "y_inv", "x_inv_density", "poisson_y_fwd", "poisson_z",
               "poisson_y_inv"]

@@ -28,7 +28,12 @@ BUILD = os.path.join(PKG, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("WDM_OFFLOAD_ARCH", "gfx950")
 
-HIP_FLAGS = ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC",
+# -ffp-contract=off: a multiply is fused into an add only where the source
+# writes fmaT.  Left to the compiler, contraction is decided per kernel, so two
+# kernels that evaluate the same expression (the density of the fused inverse
+# pass and of the standalone one, a batched pass and its single-plan twin)
+# could round it differently; with it off they agree bit for bit.
+HIP_FLAGS = ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-ffp-contract=off",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-command-line-argument"]
 EMU_FLAGS = ["-O1", "-g", "-std=c++17", "-fPIC", "-fsanitize=address", "-fno-omit-frame-pointer",

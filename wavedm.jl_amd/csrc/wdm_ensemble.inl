@@ -6,11 +6,15 @@
 // (factorisation N1 x N2, waves per workgroup; wdm_ops.hpp) on trial steps of
 // the plan's own state, and keep the fastest per axis.
 //
-// wdm_ensemble_*: many independent plans (uneven grids, each its own stream)
-// stepped together.  One step of every member is captured once into a HIP
-// graph (fork from a root stream, each member's 8 pass kernels on its own
-// stream, join), so a step of the whole ensemble is one hipGraphLaunch and
-// the members' kernels run concurrently, filling the GPU with small grids.
+// wdm_ensemble_*: many independent plans (uneven grids) stepped together.
+// Members of one shape, precision, pass table and potential kind form a
+// batch: each pass of the step is one launch over all of them (the member
+// index in the grid's y, each member's arrays and factor tables read from a
+// small device table), 8 launches per shape instead of 8 per member.  Members
+// that fit no batch step on their own streams.  One step of the ensemble
+// (every batch on its first member's stream and every other member on its
+// own, forked from and joined into a root stream) is captured once into a HIP
+// graph and launched once per step.
 
 namespace {
 
@@ -104,6 +108,17 @@ struct wdm_ensemble {
     hipGraphExec_t exec = nullptr;
     std::vector<double> g_dt, g_dtk;  // the step the graph holds
     std::vector<unsigned> g_gen;      // members' configuration at capture
+    // batches of the current step and the device tables of their members'
+    // arguments (separate for the graph, which keeps pointing at them, and
+    // for eager steps)
+    struct Batch {
+        std::vector<int> idx;  // member indices
+        int vk = 0;
+        size_t cols_off = 0, x_off = 0;  // byte offsets of the batch's tables in args
+    };
+    std::vector<Batch> batches;
+    std::vector<char> solo;  // 1: member steps on its own stream
+    DBuf g_args, e_args;
 };
 
 namespace {
@@ -130,6 +145,152 @@ bool time_dependent_v(const wdm_plan* p) {
     return ((p->vflags & V_QUADRUPOLE) && p->quad[9] != 0.0) || (p->vflags & V_TIDAL);
 }
 
+// number of column passes of a batched step: psi y FWD, z drift, y INV,
+// Poisson y FWD, z, y INV
+constexpr int kBatchCols = 6;
+
+// a member can join a batch: the reference-split step with the fused kernels
+// of the default build, no per-step host work
+bool batchable(const wdm_plan* p) {
+    return !p->ledger && !time_dependent_v(p) && !p->timer && !p->zspec_out;
+}
+
+// group the members into batches of >= 2 (same shape, precision, pass tables,
+// gravity, absorber and potential kind); the rest step alone
+void ens_group(wdm_ensemble* e) {
+    const size_t nm = e->m.size();
+    e->batches.clear();
+    e->solo.assign(nm, 1);
+    std::vector<char> used(nm, 0);
+    for (size_t i = 0; i < nm; ++i) {
+        if (used[i] || !batchable(e->m[i])) continue;
+        const wdm_plan* a = e->m[i];
+        wdm_ensemble::Batch b;
+        b.idx.push_back(int(i));
+        b.vk = vkind_of(make_vmodel(a, 0.0).flags);
+        for (size_t j = i + 1; j < nm; ++j) {
+            const wdm_plan* c = e->m[j];
+            if (used[j] || !batchable(c)) continue;
+            bool same = c->precision == a->precision && c->nx == a->nx && c->ny == a->ny && c->nz == a->nz &&
+                        c->hpitch == a->hpitch && c->self_gravity == a->self_gravity &&
+                        c->border_on == a->border_on && vkind_of(make_vmodel(c, 0.0).flags) == b.vk;
+            for (int x = 0; x < 3 && same; ++x) same = c->ax[x].ops == a->ax[x].ops;
+            if (same) b.idx.push_back(int(j));
+        }
+        if (b.idx.size() < 2) continue;
+        for (int k : b.idx) {
+            used[size_t(k)] = 1;
+            e->solo[size_t(k)] = 0;
+        }
+        e->batches.push_back(b);
+    }
+}
+
+// the device tables of every batch's member arguments for one step (kick at
+// t0, time-independent potentials), built on the host and uploaded into buf
+template <class T>
+size_t batch_table_bytes(const wdm_ensemble::Batch& b) {
+    return (sizeof(ColMember<T>) * kBatchCols + sizeof(XMember<T>)) * b.idx.size();
+}
+template <class T>
+void batch_fill(wdm_ensemble* e, wdm_ensemble::Batch& b, const double* dt, const double* dtk, double t0,
+                std::vector<char>& host, size_t off) {
+    const size_t nb = b.idx.size();
+    b.cols_off = off;
+    b.x_off = off + sizeof(ColMember<T>) * kBatchCols * nb;
+    ColMember<T>* cm = reinterpret_cast<ColMember<T>*>(host.data() + b.cols_off);
+    XMember<T>* xm = reinterpret_cast<XMember<T>*>(host.data() + b.x_off);
+    for (size_t k = 0; k < nb; ++k) {
+        const size_t i = size_t(b.idx[k]);
+        wdm_plan* p = e->m[i];
+        cx<T>* psi = p->psi.as<cx<T>>();
+        cx<T>* hat = p->self_gravity ? p->rhohat.as<cx<T>>() : nullptr;
+        const KTabs none = {nullptr, nullptr, nullptr, 0.0, nullptr, nullptr};
+        const KTabs drift = {p->ax[0].drift.p, p->ax[1].drift.p, p->ax[2].drift.p, 1.0 / double(p->ncell), nullptr,
+                             nullptr};
+        const double ncell = double(p->nx) * double(p->ny) * double(p->nz);
+        const KTabs pois = {p->ax[0].k2.p, p->ax[1].k2.p, p->ax[2].k2.p, -p->four_pi_G / ncell, nullptr, nullptr};
+        cm[0 * nb + k] = {psi, none};
+        cm[1 * nb + k] = {psi, drift};
+        cm[2 * nb + k] = {psi, none};
+        cm[3 * nb + k] = {hat, none};
+        cm[4 * nb + k] = {hat, pois};
+        cm[5 * nb + k] = {hat, none};
+        XMember<T> x;
+        std::memset(&x, 0, sizeof(x));
+        x.psi = psi;
+        x.hat = hat;
+        x.partials = p->partials.as<double>();
+        x.border = p->border_on ? p->border.as<double>() : nullptr;
+        x.ka = make_kick<T>(p, dtk[i], t0);
+        xm[k] = x;
+    }
+    (void)dt;
+}
+
+template <class T>
+void batch_issue(wdm_ensemble* e, const wdm_ensemble::Batch& b, const char* args, hipStream_t s) {
+    typedef typename TwEntry<T>::type TwE;
+    const wdm_plan* p = e->m[size_t(b.idx[0])];
+    const int nmem = int(b.idx.size());
+    const AxisOps<T>* ox = ops_of<T>(p, 0);
+    const AxisOps<T>* oy = ops_of<T>(p, 1);
+    const AxisOps<T>* oz = ops_of<T>(p, 2);
+    const ColMember<T>* cm = reinterpret_cast<const ColMember<T>*>(args + b.cols_off);
+    const XMember<T>* xm = reinterpret_cast<const XMember<T>*>(args + b.x_off);
+    const int nx = p->nx, ny = p->ny, nz = p->nz, hp = p->hpitch, nh = nx / 2 + 1;
+    const TwE* twx = p->ax[0].tw.as<TwE>();
+    const TwE* twy = p->ax[1].tw.as<TwE>();
+    const TwE* twz = p->ax[2].tw.as<TwE>();
+    ox->x_kick_b(b.vk, xm, nmem, ny * nz, hp, twx, s);
+    if (ny > 1) oy->col_b(COL_FWD, KF_NONE, cm + 0 * nmem, nmem, nx, nz, unsigned(nx), unsigned(nx * ny), twy, s);
+    oz->col_b(COL_FMI, KF_DRIFT, cm + 1 * nmem, nmem, nx, ny, unsigned(nx * ny), unsigned(nx), twz, s);
+    if (ny > 1) oy->col_b(COL_INV, KF_NONE, cm + 2 * nmem, nmem, nx, nz, unsigned(nx), unsigned(nx * ny), twy, s);
+    ox->x_inv_dens_b(xm, nmem, ny * nz, hp, twx, s);
+    if (p->self_gravity) {
+        if (ny > 1)
+            oy->col_b(COL_FWD, KF_PNONE, cm + 3 * nmem, nmem, nh, nz, unsigned(hp), unsigned(hp * ny), twy, s);
+        oz->col_b(COL_FMI, KF_POISSON, cm + 4 * nmem, nmem, nh, ny, unsigned(hp * ny), unsigned(hp), twz, s);
+        if (ny > 1)
+            oy->col_b(COL_INV, KF_PNONE, cm + 5 * nmem, nmem, nh, nz, unsigned(hp), unsigned(hp * ny), twy, s);
+    }
+    hip_check(hipGetLastError(), "batched step");
+}
+
+// group the members, build and upload the batches' argument tables into buf
+void ens_batches(wdm_ensemble* e, const double* dt, const double* dtk, double t0, DBuf& buf) {
+    ens_group(e);
+    size_t total = 0;
+    for (auto& b : e->batches) {
+        const wdm_plan* p = e->m[size_t(b.idx[0])];
+        total += p->precision == WDM_F32 ? batch_table_bytes<float>(b) : batch_table_bytes<double>(b);
+        total = (total + 255) / 256 * 256;
+    }
+    if (!total) return;
+    std::vector<char> host(total, 0);
+    size_t off = 0;
+    for (auto& b : e->batches) {
+        const wdm_plan* p = e->m[size_t(b.idx[0])];
+        if (p->precision == WDM_F32) {
+            batch_fill<float>(e, b, dt, dtk, t0, host, off);
+            off += batch_table_bytes<float>(b);
+        } else {
+            batch_fill<double>(e, b, dt, dtk, t0, host, off);
+            off += batch_table_bytes<double>(b);
+        }
+        off = (off + 255) / 256 * 256;
+    }
+    hip_check(hipMemcpy(buf.get<char>(total), host.data(), total, hipMemcpyHostToDevice), "batch tables");
+}
+
+// host-side state of a batched member after a step (as do_step leaves it)
+void batch_member_done(wdm_plan* p, double t) {
+    p->kspace = false;
+    p->mass_valid = p->self_gravity;
+    p->phi_valid = p->self_gravity;
+    p->last_kick_t = t;
+}
+
 void ens_prepare(wdm_ensemble* e, const double* dt) {
     for (size_t i = 0; i < e->m.size(); ++i) {
         wdm_plan* p = e->m[i];
@@ -142,15 +303,29 @@ void ens_prepare(wdm_ensemble* e, const double* dt) {
 
 // step s of every member (kick at t0 + s dt_i) on its own stream, forked from
 // and joined into root
-void ens_issue(wdm_ensemble* e, const double* dt, const double* dtk, double t0, int s) {
+void ens_issue(wdm_ensemble* e, const double* dt, const double* dtk, double t0, int s, const DBuf& args) {
     hip_check(hipEventRecord(e->fork, e->root), "fork");
     for (size_t i = 0; i < e->m.size(); ++i) {
+        if (!e->solo[i]) continue;
         wdm_plan* p = e->m[i];
         hip_check(hipStreamWaitEvent(p->stream, e->fork, 0), "fork wait");
         DISPATCH(p, do_step, p, 1, dt[i], dtk[i], t0 + s * dt[i], WDM_SPLIT_REFERENCE);
         hip_check(hipEventRecord(e->join[i], p->stream), "join");
-        hip_check(hipStreamWaitEvent(e->root, e->join[i], 0), "join wait");
     }
+    // each batch on its first member's stream, so the batches overlap too
+    for (const auto& b : e->batches) {
+        const size_t i0 = size_t(b.idx[0]);
+        wdm_plan* p = e->m[i0];
+        hip_check(hipStreamWaitEvent(p->stream, e->fork, 0), "fork wait");
+        if (p->precision == WDM_F32) batch_issue<float>(e, b, args.as<char>(), p->stream);
+        else batch_issue<double>(e, b, args.as<char>(), p->stream);
+        hip_check(hipEventRecord(e->join[i0], p->stream), "join");
+        for (int k : b.idx) batch_member_done(e->m[size_t(k)], t0 + s * dt[size_t(k)]);
+    }
+    for (size_t i = 0; i < e->m.size(); ++i)
+        if (e->solo[i]) hip_check(hipStreamWaitEvent(e->root, e->join[i], 0), "join wait");
+    for (const auto& b : e->batches)
+        hip_check(hipStreamWaitEvent(e->root, e->join[size_t(b.idx[0])], 0), "join wait");
 }
 
 void ens_step(wdm_ensemble* e, int nsteps, const double* dt, const double* dtk, double t0, bool use_graph) {
@@ -175,9 +350,10 @@ void ens_step(wdm_ensemble* e, int nsteps, const double* dt, const double* dtk, 
             // must start with that Poisson solve
             for (wdm_plan* p : e->m)
                 if (p->ledger) p->phi_valid = false;
+            ens_batches(e, dt, dtk, t0, e->g_args);
             hip_check(hipStreamBeginCapture(e->root, hipStreamCaptureModeRelaxed), "begin capture");
             try {
-                ens_issue(e, dt, dtk, t0, 0);
+                ens_issue(e, dt, dtk, t0, 0, e->g_args);
             } catch (...) {
                 hipGraph_t g = nullptr;
                 hipStreamEndCapture(e->root, &g);
@@ -197,7 +373,8 @@ void ens_step(wdm_ensemble* e, int nsteps, const double* dt, const double* dtk, 
 #else
     (void)use_graph;
 #endif
-    for (int s = 0; s < nsteps; ++s) ens_issue(e, dt, dtk, t0, s);
+    ens_batches(e, dt, dtk, t0, e->e_args);
+    for (int s = 0; s < nsteps; ++s) ens_issue(e, dt, dtk, t0, s, e->e_args);
     hip_check(hipStreamSynchronize(e->root), "ensemble sync");
 }
 
@@ -251,6 +428,8 @@ int wdm_ensemble_destroy(wdm_ensemble* e) {
     if (!e) return WDM_OK;
     if (e->root) hipStreamSynchronize(e->root);
     ens_drop_graph(e);
+    e->g_args.reset();
+    e->e_args.reset();
     for (hipEvent_t ev : e->join)
         if (ev) hipEventDestroy(ev);
     if (e->fork) hipEventDestroy(e->fork);
@@ -266,6 +445,15 @@ int wdm_ensemble_step(wdm_ensemble* e, int nsteps, const double* dt, const doubl
             return fail_code(nullptr, WDM_ERR_ARG, "wdm_ensemble_step: non-finite dt");
     if (nsteps == 0) return WDM_OK;
     return ens_guarded([&] { ens_step(e, nsteps, dt, dt_kick, t0, use_graph != 0); });
+}
+
+int wdm_ensemble_layout(wdm_ensemble* e, int* nbatches, int* nsolo) {
+    if (!e || !nbatches || !nsolo) return fail_code(nullptr, WDM_ERR_ARG, "wdm_ensemble_layout: NULL argument");
+    *nbatches = int(e->batches.size());
+    int n = 0;
+    for (char c : e->solo) n += c;
+    *nsolo = e->solo.empty() ? int(e->m.size()) : n;
+    return WDM_OK;
 }
 
 int wdm_ensemble_autotune(wdm_ensemble* e, int trial_steps, const double* dt) {

@@ -105,8 +105,11 @@ WDM_FI KfLane<KF, T> kf_lane(const KTabs& kt, int c, int o) {
         const cx<double> a = static_cast<const cx<double>*>(kt.col)[c];
         const cx<double> b = static_cast<const cx<double>*>(kt.oth)[o];
         const double sc = kt.norm ? kt.scale * kt.norm[0] : kt.scale;
-        l.ar = (a.x * b.x - a.y * b.y) * sc;
-        l.ai = (a.x * b.y + a.y * b.x) * sc;
+        // explicit fmas (here and in apply_kf): every kernel that applies the
+        // factor (COL_FMI, COL_FMIS, COL_MI, batched or not) rounds it the
+        // same way, whatever the compiler would contract
+        l.ar = fmaT(a.x, b.x, -(a.y * b.y)) * sc;
+        l.ai = fmaT(a.x, b.y, a.y * b.x) * sc;
     } else if constexpr (KF == KF_POISSON) {
         l.kco = static_cast<const T*>(kt.col)[c] + static_cast<const T*>(kt.oth)[o];
         l.scale = T(kt.scale);
@@ -120,7 +123,7 @@ template <int KF, class P, class T>
 WDM_FI cx<T> apply_kf(cx<T> v, const KfLane<KF, T>& l, const KTabs& kt, int k) {
     if constexpr (KF == KF_DRIFT) {
         const cx<double> c = static_cast<const cx<double>*>(kt.line)[k];
-        return mul_dyn<P>(v, l.ar * c.x - l.ai * c.y, l.ar * c.y + l.ai * c.x);
+        return mul_dyn<P>(v, fmaT(l.ar, c.x, -(l.ai * c.y)), fmaT(l.ar, c.y, l.ai * c.x));
     } else if constexpr (KF == KF_POISSON) {
         const T s = l.kco + static_cast<const T*>(kt.line)[k];
         if constexpr (std::is_same<T, double>::value) {
@@ -359,15 +362,36 @@ WDM_FI unsigned xcd_block() {
     return blockIdx.x;
 #endif
 }
+// b: the workgroup's index within its pass (blockIdx.x, or the member-local
+// index of a batched launch)
 template <class F, int NW>
-WDM_FI LaneMap lane_map() {
+WDM_FI LaneMap lane_map(unsigned b) {
     LaneMap m;
     m.lane = int(threadIdx.x) & (kWave - 1);
     m.wv = int(threadIdx.x) / kWave;
     m.j = m.lane % F::N2;
     m.lw = m.lane / F::N2;
-    m.w = int(blockIdx.x) * NW + m.wv;
+    m.w = int(b) * NW + m.wv;
     return m;
+}
+template <class F, int NW>
+WDM_FI LaneMap lane_map() {
+    return lane_map<F, NW>(blockIdx.x);
+}
+
+// Batched launches (an ensemble's members of one shape): grid (g, nmembers),
+// the workgroup of linear id L = y g + x.  The XCD renumbering of xcd_block()
+// is applied to L over the whole grid, then split into (member, index within
+// the member's pass of g workgroups).
+WDM_FI void batch_block(unsigned& member, unsigned& b) {
+    const unsigned g = gridDim.x, total = gridDim.x * gridDim.y;
+    unsigned L = blockIdx.y * g + blockIdx.x;
+#if WDM_XCD_REMAP
+    const unsigned q = total / 8u;
+    if (L < q * 8u) L = (L % 8u) * q + L / 8u;
+#endif
+    member = L / g;
+    b = L % g;
 }
 
 // Deterministic wave sum/max of per-lane FP64 values through the wave's LDS
@@ -591,11 +615,15 @@ void k_colp(cx<T>* __restrict__ data, int ncols, int nother, unsigned sline, uns
 // Lines along an axis with element stride `sline`; columns are the contiguous
 // x index c in [0, ncols); `nother` positions of the remaining axis at stride
 // `sother`.  Each wave owns LPW adjacent columns at one `other` position.
-// minimum waves per SIMD asked of the compiler for a pass kernel: the fused
-// FFT * factor * IFFT passes in FP32 sit just above the 168-register line of
-// three waves per SIMD
+// minimum waves per SIMD asked of the compiler for a pass kernel.  The fused
+// FFT * drift * IFFT pass in FP32 needs 176 registers for two waves per SIMD;
+// capped at 168 for three (a 40-byte spill) it runs 9 us faster at 384^3
+// (272 vs 281 us).  The Poisson z pass (182 registers) is slower capped.
 #ifndef WDM_MINW_FMI_F32
-#define WDM_MINW_FMI_F32 1
+#define WDM_MINW_FMI_F32 3
+#endif
+#ifndef WDM_MINW_PFMI_F32
+#define WDM_MINW_PFMI_F32 1
 #endif
 #ifndef WDM_MINW_XINV_F32
 #define WDM_MINW_XINV_F32 1
@@ -603,10 +631,10 @@ void k_colp(cx<T>* __restrict__ data, int ncols, int nother, unsigned sline, uns
 #ifndef WDM_MINW_COL_F32
 #define WDM_MINW_COL_F32 1
 #endif
-template <class T, int MODE>
+template <class T, int MODE, int KF>
 constexpr int col_min_waves() {
     if (!std::is_same<T, float>::value) return 1;
-    if (MODE == COL_FMI) return WDM_MINW_FMI_F32;
+    if (MODE == COL_FMI) return KF == KF_POISSON ? WDM_MINW_PFMI_F32 : WDM_MINW_FMI_F32;
     if (MODE == COL_FWD || MODE == COL_INV) return WDM_MINW_COL_F32;
     return 1;
 }
@@ -618,17 +646,17 @@ constexpr int col_min_waves() {
 #define WDM_COL_EARLY_LOADS 0
 #endif
 
+// One column pass of workgroup `bid` (already XCD-renumbered).
 template <class T, class P, class F, int MODE, int KF, int NW>
-WDM_KERNEL_W(NW * 64, (col_min_waves<T, MODE>()))
-void k_col(cx<T>* __restrict__ data, int ncols, int nother, unsigned sline, unsigned sother,
-           const typename TwEntry<T>::type* __restrict__ tw, KTabs kt) {
+WDM_FI void col_body(cx<T>* __restrict__ data, int ncols, int nother, unsigned sline, unsigned sother,
+                     const typename TwEntry<T>::type* __restrict__ tw, KTabs kt, unsigned bid) {
     constexpr int N1 = F::N1, N2 = F::N2, M = F::M;
     constexpr bool EARLY = WDM_COL_EARLY_LOADS && MODE != COL_MI;
     WDM_SHARED cx<T> xbuf[NW][F::XBUF];
     WDM_SHARED typename TwEntry<T>::type tw_s[F::N];
-    const LaneMap m = lane_map<F, NW>();
+    const LaneMap m = lane_map<F, NW>(bid);
     const int ntiles = (ncols + F::LPW - 1) / F::LPW;
-    const int wl = int(xcd_block()) * NW + m.wv;
+    const int wl = int(bid) * NW + m.wv;
     const int o = wl / ntiles, tile = wl % ntiles;
     const int c = tile * F::LPW + m.lw;
     const bool ok = (o < nother) && (c < ncols);
@@ -703,6 +731,28 @@ void k_col(cx<T>* __restrict__ data, int ncols, int nother, unsigned sline, unsi
     }
 }
 
+template <class T, class P, class F, int MODE, int KF, int NW>
+WDM_KERNEL_W(NW * 64, (col_min_waves<T, MODE, KF>()))
+void k_col(cx<T>* __restrict__ data, int ncols, int nother, unsigned sline, unsigned sother,
+           const typename TwEntry<T>::type* __restrict__ tw, KTabs kt) {
+    col_body<T, P, F, MODE, KF, NW>(data, ncols, nother, sline, sother, tw, kt, xcd_block());
+}
+
+// a member of a batched column pass: its array and factor tables
+template <class T>
+struct ColMember {
+    cx<T>* data;
+    KTabs kt;
+};
+template <class T, class P, class F, int MODE, int KF, int NW>
+WDM_KERNEL_W(NW * 64, (col_min_waves<T, MODE, KF>()))
+void k_col_b(const ColMember<T>* __restrict__ mem, int ncols, int nother, unsigned sline, unsigned sother,
+             const typename TwEntry<T>::type* __restrict__ tw) {
+    unsigned member, b;
+    batch_block(member, b);
+    col_body<T, P, F, MODE, KF, NW>(mem[member].data, ncols, nother, sline, sother, tw, mem[member].kt, b);
+}
+
 // ------------------------------------------------------------------ x-passes
 struct KickArgs {
     double mh_dt;        // (m/hbar) * dt_kick
@@ -732,14 +782,13 @@ inline int vkind_of(int flags) {
 }
 
 template <class T, class P, class F, bool FWD, int NW, int VK>
-WDM_KERNEL(NW * 64)
-void k_x_kick(cx<T>* __restrict__ psi, const cx<T>* __restrict__ phihat, int nlines, int hpitch,
-              const typename TwEntry<T>::type* __restrict__ tw, KickArgs ka) {
+WDM_FI void x_kick_body(cx<T>* __restrict__ psi, const cx<T>* __restrict__ phihat, int nlines, int hpitch,
+                        const typename TwEntry<T>::type* __restrict__ tw, const KickArgs& ka, unsigned bid) {
     constexpr int N1 = F::N1, N2 = F::N2, M = F::M, N = F::N;
     WDM_SHARED cx<T> xbuf[NW][F::XBUF];
     WDM_SHARED typename TwEntry<T>::type tw_s[F::N];
     tw = stage_table<F::N>(tw_s, tw);
-    const LaneMap m = lane_map<F, NW>();
+    const LaneMap m = lane_map<F, NW>(bid);
     const int l = m.w * F::LPW + m.lw;
     const bool ok = l < nlines;
     const int lc = ok ? l : 0;
@@ -823,6 +872,32 @@ void k_x_kick(cx<T>* __restrict__ psi, const cx<T>* __restrict__ phihat, int nli
     }
 }
 
+template <class T, class P, class F, bool FWD, int NW, int VK>
+WDM_KERNEL(NW * 64)
+void k_x_kick(cx<T>* __restrict__ psi, const cx<T>* __restrict__ phihat, int nlines, int hpitch,
+              const typename TwEntry<T>::type* __restrict__ tw, KickArgs ka) {
+    x_kick_body<T, P, F, FWD, NW, VK>(psi, phihat, nlines, hpitch, tw, ka, blockIdx.x);
+}
+
+// a member of a batched x pass: its arrays, kick arguments and absorber
+template <class T>
+struct XMember {
+    cx<T>* psi;
+    cx<T>* hat;           // Phi_hat in (kick), rho_hat out (inverse pass); null without self-gravity
+    double* partials;     // per-wave mass partials of the inverse pass
+    const double* border; // length-nx absorber or null
+    KickArgs ka;
+};
+template <class T, class P, class F, int NW, int VK>
+WDM_KERNEL(NW * 64)
+void k_x_kick_b(const XMember<T>* __restrict__ mem, int nlines, int hpitch,
+                const typename TwEntry<T>::type* __restrict__ tw) {
+    unsigned member, b;
+    batch_block(member, b);
+    const XMember<T>& mm = mem[member];
+    x_kick_body<T, P, F, true, NW, VK>(mm.psi, mm.hat, nlines, hpitch, tw, mm.ka, b);
+}
+
 // rho = |psi|^2 of real-space values v[] (positions x = N2*kq + j), forward R2C
 // along x into the half-spectrum row h, plus FP64 sum/max partials per wave.
 template <class T, class F, int NW>
@@ -865,15 +940,14 @@ WDM_FI void density_r2c(const cx<T>* v, bool ok, const LaneMap& m, cx<T>* xb, cx
 // then rho = |psi|^2 -> R2C_x -> rho_hat.  `border` is a length-Nx FP64
 // vector or null (src/solver/KDK.jl:95-96; broadcast along x, Q4).
 template <class T, class P, class F, int NW>
-WDM_KERNEL_W(NW * 64, (std::is_same<T, float>::value ? WDM_MINW_XINV_F32 : 1))
-void k_x_inv_dens(cx<T>* __restrict__ psi, cx<T>* __restrict__ rhohat, int nlines, int hpitch,
-                  const typename TwEntry<T>::type* __restrict__ tw, const double* __restrict__ border,
-                  double* __restrict__ partials) {
+WDM_FI void x_inv_dens_body(cx<T>* __restrict__ psi, cx<T>* __restrict__ rhohat, int nlines, int hpitch,
+                            const typename TwEntry<T>::type* __restrict__ tw, const double* __restrict__ border,
+                            double* __restrict__ partials, unsigned bid) {
     constexpr int N1 = F::N1, N2 = F::N2, M = F::M, N = F::N;
     WDM_SHARED cx<T> xbuf[NW][F::XBUF];
     WDM_SHARED typename TwEntry<T>::type tw_s[F::N];
     tw = stage_table<F::N>(tw_s, tw);
-    const LaneMap m = lane_map<F, NW>();
+    const LaneMap m = lane_map<F, NW>(bid);
     const int l = m.w * F::LPW + m.lw;
     const bool ok = l < nlines;
     const int lc = ok ? l : 0;
@@ -898,6 +972,22 @@ void k_x_inv_dens(cx<T>* __restrict__ psi, cx<T>* __restrict__ rhohat, int nline
     if (rhohat) {
         density_r2c<T, F, NW>(v, ok, m, xb, &xbuf[m.wv][0], tw, rhohat, unsigned(lc) * unsigned(hpitch), partials);
     }
+}
+template <class T, class P, class F, int NW>
+WDM_KERNEL_W(NW * 64, (std::is_same<T, float>::value ? WDM_MINW_XINV_F32 : 1))
+void k_x_inv_dens(cx<T>* __restrict__ psi, cx<T>* __restrict__ rhohat, int nlines, int hpitch,
+                  const typename TwEntry<T>::type* __restrict__ tw, const double* __restrict__ border,
+                  double* __restrict__ partials) {
+    x_inv_dens_body<T, P, F, NW>(psi, rhohat, nlines, hpitch, tw, border, partials, blockIdx.x);
+}
+template <class T, class P, class F, int NW>
+WDM_KERNEL_W(NW * 64, (std::is_same<T, float>::value ? WDM_MINW_XINV_F32 : 1))
+void k_x_inv_dens_b(const XMember<T>* __restrict__ mem, int nlines, int hpitch,
+                    const typename TwEntry<T>::type* __restrict__ tw) {
+    unsigned member, b;
+    batch_block(member, b);
+    const XMember<T>& mm = mem[member];
+    x_inv_dens_body<T, P, F, NW>(mm.psi, mm.hat, nlines, hpitch, tw, mm.border, mm.partials, b);
 }
 
 // rho_hat (and sum/max partials) from psi in real space (start of a run, or

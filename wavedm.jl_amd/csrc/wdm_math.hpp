@@ -30,7 +30,6 @@ template <int S, class T> WDM_HDFI cx<T> mul_i(cx<T> a) {
     if constexpr (S > 0) return mk(-a.y, a.x);
     else return mk(a.y, -a.x);
 }
-template <class T> WDM_HDFI T norm2(cx<T> a) { return a.x * a.x + a.y * a.y; }
 
 // Element access at a 32-bit index from a wave-uniform base pointer: the byte
 // offset is formed in 32 bits so the backend can use scalar-base addressing
@@ -46,6 +45,11 @@ WDM_HDFI void stg(T* base, unsigned idx, const T& v) {
 
 WDM_HDFI float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 WDM_HDFI double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// |a|^2 with an explicit fma: the same bits in every kernel that evaluates it
+// (left to contraction, two kernels computing the density of the same psi
+// could round it differently)
+template <class T> WDM_HDFI T norm2(cx<T> a) { return fmaT(a.x, a.x, a.y * a.y); }
 
 struct Plain {};
 struct Comp {};

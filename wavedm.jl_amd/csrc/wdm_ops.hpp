@@ -100,6 +100,13 @@ struct AxisOps {
                   double kscale, double* partials, hipStream_t s);
     void (*col_red)(const cx<T>* data, int ncols, int nother, unsigned sline, unsigned sother, const TwE* tw,
                     double kscale, double* partials, hipStream_t s);
+    // batched passes over nmem members of this shape (the step's passes only:
+    // the x kick with FFT, the column modes of the step and the inverse x
+    // pass with density); the members' arguments are device arrays
+    void (*col_b)(int mode, int kf, const ColMember<T>* mem, int nmem, int ncols, int nother, unsigned sline,
+                  unsigned sother, const TwE* tw, hipStream_t s);
+    void (*x_kick_b)(int vk, const XMember<T>* mem, int nmem, int nlines, int hpitch, const TwE* tw, hipStream_t s);
+    void (*x_inv_dens_b)(const XMember<T>* mem, int nmem, int nlines, int hpitch, const TwE* tw, hipStream_t s);
 };
 
 // Grid sizes (in workgroups of NW waves).
@@ -152,7 +159,7 @@ struct Launch {
                        dim3(col_blocks<NW * WDM_COL_TPW, F>(ncols, nother)), dim3(NW * 64), s, d, ncols, nother, sl,
                        so, tw, kt);
         } else {
-            constexpr int W = (MODE == COL_FMI && col_min_waves<T, MODE>() > 1) ? WDM_NW_FMI : NW;
+            constexpr int W = (MODE == COL_FMI && col_min_waves<T, MODE, KF>() > 1) ? WDM_NW_FMI : NW;
             WDM_LAUNCH((k_col<T, PsiP, F, MODE, KF, W>), dim3(col_blocks<W, F>(ncols, nother)), dim3(W * 64), s,
                        d, ncols, nother, sl, so, tw, kt);
         }
@@ -167,7 +174,7 @@ struct Launch {
                        dim3(col_blocks<NW * WDM_COL_TPW, F>(ncols, nother)), dim3(NW * 64), s, d, ncols, nother, sl,
                        so, tw, kt);
         } else {
-            constexpr int W = (MODE == COL_FMI && col_min_waves<T, MODE>() > 1) ? WDM_NW_FMI : NW;
+            constexpr int W = (MODE == COL_FMI && col_min_waves<T, MODE, KF>() > 1) ? WDM_NW_FMI : NW;
             WDM_LAUNCH((k_col<T, PoisP, F, MODE, KF, W>), dim3(col_blocks<W, F>(ncols, nother)), dim3(W * 64), s,
                        d, ncols, nother, sl, so, tw, kt);
         }
@@ -265,10 +272,51 @@ struct Launch {
         WDM_LAUNCH((k_col_red<T, PsiP, F, NW>), dim3(col_blocks<NW, F>(ncols, nother)), dim3(NW * 64), s, d, ncols,
                    nother, sl, so, tw, kscale, partials);
     }
+    // batched launches: the same kernel bodies as k_col / k_x_kick /
+    // k_x_inv_dens of the default build, member index in the grid's y
+    template <class PP, int MODE, int KF>
+    static void colb_t(const ColMember<T>* mem, int nmem, int ncols, int nother, unsigned sl, unsigned so,
+                       const TwE* tw, hipStream_t s) {
+        constexpr int W = (MODE == COL_FMI && col_min_waves<T, MODE, KF>() > 1) ? WDM_NW_FMI : NW;
+        WDM_LAUNCH((k_col_b<T, PP, F, MODE, KF, W>), dim3(col_blocks<W, F>(ncols, nother), unsigned(nmem)),
+                   dim3(W * 64), s, mem, ncols, nother, sl, so, tw);
+    }
+    static void col_b(int mode, int kf, const ColMember<T>* mem, int nmem, int ncols, int nother, unsigned sl,
+                      unsigned so, const TwE* tw, hipStream_t s) {
+        if (kf == KF_POISSON) {
+            colb_t<PoisP, COL_FMI, KF_POISSON>(mem, nmem, ncols, nother, sl, so, tw, s);
+        } else if (kf == KF_PNONE) {
+            if (mode == COL_FWD) colb_t<PoisP, COL_FWD, KF_PNONE>(mem, nmem, ncols, nother, sl, so, tw, s);
+            else colb_t<PoisP, COL_INV, KF_PNONE>(mem, nmem, ncols, nother, sl, so, tw, s);
+        } else if (mode == COL_FWD) {
+            colb_t<PsiP, COL_FWD, KF_NONE>(mem, nmem, ncols, nother, sl, so, tw, s);
+        } else if (mode == COL_INV) {
+            colb_t<PsiP, COL_INV, KF_NONE>(mem, nmem, ncols, nother, sl, so, tw, s);
+        } else {
+            colb_t<PsiP, COL_FMI, KF_DRIFT>(mem, nmem, ncols, nother, sl, so, tw, s);
+        }
+    }
+    template <int VK>
+    static void x_kick_bv(const XMember<T>* mem, int nmem, int nlines, int hp, const TwE* tw, hipStream_t s) {
+        WDM_LAUNCH((k_x_kick_b<T, PsiP, F, NW, VK>), dim3(x_blocks<NW, F>(nlines), unsigned(nmem)), dim3(NW * 64), s,
+                   mem, nlines, hp, tw);
+    }
+    static void x_kick_b(int vk, const XMember<T>* mem, int nmem, int nlines, int hp, const TwE* tw, hipStream_t s) {
+        switch (vk) {
+            case VK_NONE: x_kick_bv<VK_NONE>(mem, nmem, nlines, hp, tw, s); break;
+            case VK_QUAD: x_kick_bv<VK_QUAD>(mem, nmem, nlines, hp, tw, s); break;
+            case VK_HERN: x_kick_bv<VK_HERN>(mem, nmem, nlines, hp, tw, s); break;
+            default: x_kick_bv<VK_ANY>(mem, nmem, nlines, hp, tw, s); break;
+        }
+    }
+    static void x_inv_dens_b(const XMember<T>* mem, int nmem, int nlines, int hp, const TwE* tw, hipStream_t s) {
+        WDM_LAUNCH((k_x_inv_dens_b<T, PsiP, F, NW>), dim3(x_blocks<NW, F>(nlines), unsigned(nmem)), dim3(NW * 64), s,
+                   mem, nlines, hp, tw);
+    }
     static const AxisOps<T>* table() {
         static const AxisOps<T> ops = {F::N, F::N1, F::N2, F::LPW, NW, &col, &x_kick, &x_inv_dens,
                                        &x_dens, &x_r2c_real, &x_c2r_real, &x_fft, &adj_xs, &adj_xg,
-                                       &x_red, &col_red};
+                                       &x_red, &col_red, &col_b, &x_kick_b, &x_inv_dens_b};
         return &ops;
     }
 };

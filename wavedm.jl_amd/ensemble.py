@@ -1,6 +1,6 @@
 """Batched ensemble of independent boxes (wdm_ensemble_*): many plans of
-uneven grids stepped together, one HIP-graph launch per step, with per-shape
-autotuned pass tables.  The reference steps one box at a time
+uneven grids stepped together (members of one shape in one launch per pass),
+one HIP-graph launch per step, with per-shape autotuned pass tables.  The reference steps one box at a time
 (src/simulation_main.jl:536-551); this is the host side of the batched form."""
 from __future__ import annotations
 
@@ -53,6 +53,14 @@ class Ensemble:
         k = self._vec(0.5 * d if dt_kick is None else dt_kick, "dt_kick")
         A.check(self.lib, None, self.lib.wdm_ensemble_step(
             self._h, int(nsteps), d.ctypes.data_as(A._DP), k.ctypes.data_as(A._DP), float(t0), 1 if graph else 0))
+
+    def layout(self):
+        """(batches, members stepping alone) of the last step: members of one
+        shape, precision, pass table and potential kind share one launch per
+        pass."""
+        nb, ns = C.c_int(0), C.c_int(0)
+        A.check(self.lib, None, self.lib.wdm_ensemble_layout(self._h, C.byref(nb), C.byref(ns)))
+        return nb.value, ns.value
 
     def autotune(self, trial_steps, dt):
         d = self._vec(dt, "dt")
