@@ -91,26 +91,6 @@ void k_adj_g(cx<T>* __restrict__ g, const cx<T>* __restrict__ psi, const double*
     }
 }
 
-// k space: G <- conj(P) G / N and partial sums of w_k Im(conj(G_new) Psi') / N
-template <class T>
-WDM_KERNEL(256)
-void k_adj_spec(cx<T>* __restrict__ G, const cx<T>* __restrict__ Psi, int nx, int ny, int nz,
-                const double* __restrict__ kx2, const double* __restrict__ ky2, const double* __restrict__ kz2,
-                double c /* (hbar/m) dt / 2 */, double inv_n, double* __restrict__ partials) {
-    WDM_SHARED double sh[1][256];
-    double acc[1] = {0.0};
-    WDM_FOR_CELLS(nx, ny, nz) {
-        const double w = c * (kx2[x] + ky2[y] + kz2[z]);
-        const cx<double> e = expmi(w);  // P; conj(P) = (e.x, -e.y)
-        const double gr = double(G[i].x), gi = double(G[i].y);
-        const double hr = (e.x * gr + e.y * gi) * inv_n, hi = (e.x * gi - e.y * gr) * inv_n;
-        G[i] = mk(T(hr), T(hi));
-        const double ar = double(Psi[i].x), ai = double(Psi[i].y);
-        acc[0] += w * (hr * ai - hi * ar);  // Im(conj(h) Psi) ; the 1/N of F g' is in h
-    }
-    block_sums<1>(acc, sh, partials);
-}
-
 // acc[j] += sum_b partials[nv*b + j] for j < nv: one block of 256 threads per
 // component, each thread a fixed strided subset (Neumaier), then a fixed-order
 // tree; deterministic.
@@ -212,8 +192,9 @@ void adj_forward(wdm_plan* p, int nsteps, double dt, double dt_kick, double t0, 
 
 // g (device, transformed along x) <- the y and z passes of the conjugate drift
 // U^H g (the x passes are fused into k_adj_xg / k_adj_xs); with Psi given,
-// also the block partials of the dt-scale spectral term into part (unfused
-// z passes)
+// the z pass also sums the dt-scale spectral term, one partial per wave of
+// the pass into part (adj_spec_waves of them)
+inline int adj_spec_waves(const wdm_plan* p, int lpw_z) { return (p->nx + lpw_z - 1) / lpw_z * p->ny; }
 template <class T>
 void adj_conj_drift_yz(wdm_plan* p, cx<T>* g, const cx<T>* Psi, double* part) {
     typedef typename TwEntry<T>::type TwE;
@@ -226,24 +207,16 @@ void adj_conj_drift_yz(wdm_plan* p, cx<T>* g, const cx<T>* Psi, double* part) {
     if (p->ny > 1)
         oy->col(COL_FWD, KF_NONE, g, p->nx, p->nz, unsigned(p->nx), unsigned(p->nx * p->ny), p->ax[1].tw.as<TwE>(),
                 none, p->stream);
-    if (!Psi) {
-        ensure_drift(p, A.dt);
-        KTabs kt = {p->ax[0].drift_conj.p, p->ax[1].drift_conj.p, p->ax[2].drift_conj.p, 1.0 / double(p->ncell),
-                    nullptr};
-        oz->col(COL_FMI, KF_NONE, g, p->nx, p->ny, unsigned(p->nx * p->ny), unsigned(p->nx), p->ax[2].tw.as<TwE>(), kt,
-                p->stream);
-    } else {
-        if (p->nz > 1)
-            oz->col(COL_FWD, KF_NONE, g, p->nx, p->ny, unsigned(p->nx * p->ny), unsigned(p->nx),
-                    p->ax[2].tw.as<TwE>(), none, p->stream);
-        const unsigned nb = adj_line_blocks(p->ny, p->nz);
-        WDM_LAUNCH((k_adj_spec<T>), dim3(nb), dim3(256), p->stream, g, Psi, p->nx, p->ny, p->nz,
-                   A.kfull[0].as<double>(), A.kfull[1].as<double>(), A.kfull[2].as<double>(),
-                   0.5 * p->hbar_over_m * A.dt, 1.0 / double(p->ncell), part);
-        if (p->nz > 1)
-            oz->col(COL_INV, KF_NONE, g, p->nx, p->ny, unsigned(p->nx * p->ny), unsigned(p->nx),
-                    p->ax[2].tw.as<TwE>(), none, p->stream);
+    ensure_drift(p, A.dt);
+    KTabs kt = {p->ax[0].drift_conj.p, p->ax[1].drift_conj.p, p->ax[2].drift_conj.p, 1.0 / double(p->ncell),
+                const_cast<cx<T>*>(Psi), nullptr};
+    if (Psi) {
+        for (int a = 0; a < 3; ++a) kt.k2[a] = A.kfull[a].as<double>();
+        kt.wc = 0.5 * p->hbar_over_m * A.dt;
+        kt.partials = part;
     }
+    oz->col(Psi ? COL_FMIA : COL_FMI, KF_DRIFT, g, p->nx, p->ny, unsigned(p->nx * p->ny), unsigned(p->nx),
+            p->ax[2].tw.as<TwE>(), kt, p->stream);
     if (p->ny > 1)
         oy->col(COL_INV, KF_NONE, g, p->nx, p->nz, unsigned(p->nx), unsigned(p->nx * p->ny), p->ax[1].tw.as<TwE>(),
                 none, p->stream);
@@ -275,10 +248,11 @@ void adj_backward(wdm_plan* p, const void* grad, int dtype, void* grad0, double*
     cx<T>* g = A.g.get<cx<T>>(n);
     // workgroup partials of every step, summed once at the end in a fixed order
     // (the result does not depend on the checkpoint interval):
-    // [nsteps][nbx][kAdjNV] parameter sums of k_adj_xs, then [nsteps][nb]
-    // dt-scale spectral sums of k_adj_spec
+    // [nsteps][nbx][kAdjNV] parameter sums of k_adj_xs, then [nsteps][nbs]
+    // dt-scale spectral sums of the conjugate drift's z pass (one per wave)
     const unsigned nbx = unsigned(((nlines + ox->lpw - 1) / ox->lpw + ox->nw - 1) / ox->nw);
-    double* part = A.part.get<double>(size_t(A.nsteps) * (size_t(kAdjNV) * nbx + nb));
+    const unsigned nbs = unsigned(adj_spec_waves(p, ops_of<T>(p, 2)->lpw));
+    double* part = A.part.get<double>(size_t(A.nsteps) * (size_t(kAdjNV) * nbx + nbs));
     double* part_spec = part + size_t(A.nsteps) * kAdjNV * nbx;
     double* acc = A.acc.get<double>(kAdjNV);
     cx<T>* tspec = want_dt ? A.tspec.get<cx<T>>(n * size_t(A.K)) : nullptr;
@@ -352,7 +326,7 @@ void adj_backward(wdm_plan* p, const void* grad, int dtype, void* grad0, double*
             xa.border = nullptr;
             xa.vm = make_vmodel(p, A.t0 + double(step) * A.dt);
             quad_rotation(p, A.t0 + double(step) * A.dt, xa.qR);
-            adj_conj_drift_yz<T>(p, g, tspec ? tspec + n * size_t(i) : nullptr, part_spec + size_t(step) * nb);
+            adj_conj_drift_yz<T>(p, g, tspec ? tspec + n * size_t(i) : nullptr, part_spec + size_t(step) * nbs);
             cx<T>* rh = p->self_gravity ? p->rhohat.as<cx<T>>() : nullptr;
             ox->adj_xs(g, pn, rh, nlines, p->hpitch, p->ax[0].tw.as<TwE>(), xa,
                        part + size_t(step) * kAdjNV * nbx, p->stream);
@@ -365,8 +339,8 @@ void adj_backward(wdm_plan* p, const void* grad, int dtype, void* grad0, double*
     WDM_LAUNCH((k_adj_accumulate), dim3(kAdjNV), dim3(256), p->stream, (const double*)part,
                int(size_t(A.nsteps) * nbx), kAdjNV, acc);
     if (want_dt)
-        WDM_LAUNCH((k_adj_accumulate), dim3(1), dim3(256), p->stream, (const double*)part_spec, int(A.nsteps * nb), 1,
-                   acc + 5);
+        WDM_LAUNCH((k_adj_accumulate), dim3(1), dim3(256), p->stream, (const double*)part_spec, int(A.nsteps * nbs),
+                   1, acc + 5);
     // outputs; restore psi_N
     double h[kAdjNV];
     hip_check(hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, p->stream), "param grads");

@@ -44,7 +44,11 @@ typedef Comp PsiPolicyF32;
 #endif
 
 // COL_FMIS: COL_FMI that also stores the forward spectrum to KTabs::spec_out
-enum ColMode { COL_FWD = 0, COL_INV = 1, COL_FMI = 2, COL_MI = 3, COL_FMIS = 4 };
+// COL_FMIA: COL_FMI that also sums the adjoint's dt-scale spectral term
+//           sum_k w_k Im(conj(h_k) Psi_k), h = the factored spectrum,
+//           Psi = KTabs::spec_out (read), w_k = KTabs::wc * k^2, one FP64 sum
+//           per wave into KTabs::partials
+enum ColMode { COL_FWD = 0, COL_INV = 1, COL_FMI = 2, COL_MI = 3, COL_FMIS = 4, COL_FMIA = 5 };
 // KF_PNONE: no factor, Plain arithmetic (the y passes of the Poisson chain)
 enum KFactor { KF_NONE = 0, KF_DRIFT = 1, KF_POISSON = 2, KF_PNONE = 3 };
 
@@ -65,8 +69,16 @@ struct KTabs {
     const void* line;
     double scale;
     void* spec_out;  // COL_FMIS: the forward spectrum (before the factor) is
-                     // also stored here, same layout as data
+                     // also stored here, same layout as data; COL_FMIA: the
+                     // spectrum Psi it reads, same layout
     const double* norm;  // KF_DRIFT: device scalar multiplying the factor (mass projection) or null
+    // COL_FMIA: k^2 of the column, other and line axes (FP64), the weight
+    // w_k = wc * (kc^2 + ko^2 + kl^2), and one partial per wave of the pass
+    // (wave w = workgroup * waves per group + wave in group, for the
+    // ceil(ncols / LPW) * nother waves that own lines)
+    const double* k2[3];
+    double wc;
+    double* partials;
 };
 
 // line-axis table entry of a factor kind
@@ -680,6 +692,11 @@ WDM_FI void col_body(cx<T>* __restrict__ data, int ncols, int nother, unsigned s
         for (int i = int(threadIdx.x); i < NL; i += int(blockDim.x)) line_s[i] = src[i];
         kt.line = line_s;
     }
+    // COL_FMIA: the line axis's k^2, staged with the tables
+    WDM_SHARED double k2l_s[MODE == COL_FMIA ? F::N : 1];
+    if constexpr (MODE == COL_FMIA) {
+        for (int i = int(threadIdx.x); i < F::N; i += int(blockDim.x)) k2l_s[i] = kt.k2[2][i];
+    }
     tw = stage_table<F::N>(tw_s, tw);
     const KfLane<KF, T> kl = kf_lane<KF, T>(kt, cc, oc);
     cx<T>* xb = &xbuf[m.wv][m.lw * F::XS];
@@ -694,7 +711,7 @@ WDM_FI void col_body(cx<T>* __restrict__ data, int ncols, int nother, unsigned s
         WDM_UNROLL
         for (int n1 = 0; n1 < N1; ++n1) v[n1] = WDM_LDG(ok, data, base + unsigned(n1) * step);
     }
-    constexpr bool FMI = MODE == COL_FMI || MODE == COL_FMIS;
+    constexpr bool FMI = MODE == COL_FMI || MODE == COL_FMIS || MODE == COL_FMIA;
     constexpr int DIR = (MODE == COL_FWD || FMI) ? -1 : +1;
     fwd2<F, DIR, P>(v, X, m.j, xb, tw);
     if constexpr (FMI) {
@@ -706,12 +723,38 @@ WDM_FI void col_body(cx<T>* __restrict__ data, int ncols, int nother, unsigned s
                 for (int k2 = 0; k2 < N2; ++k2)
                     stg(so, unsigned(oc) * sother + unsigned(cc) + unsigned((i * N2 + m.j) + N1 * k2) * sline, X[i][k2]);
         }
+        // COL_FMIA: sum_k w_k Im(conj(h_k) Psi_k) over this lane's elements
+        // (h = the factored spectrum), in FP64, a chunk of Psi at a time
+        double acc = 0.0;
+        const double kco = MODE == COL_FMIA ? kt.k2[0][cc] + kt.k2[1][oc] : 0.0;
         WDM_UNROLL
         for (int i = 0; i < M; ++i) {
             WDM_SCHED_FENCE();
+            cx<T> q[N2];
+            if constexpr (MODE == COL_FMIA) {
+                const cx<T>* ps = static_cast<const cx<T>*>(kt.spec_out);
+                WDM_UNROLL
+                for (int k2 = 0; k2 < N2; ++k2)
+                    q[k2] = WDM_LDG(ok, ps, unsigned(oc) * sother + unsigned(cc) +
+                                                unsigned((i * N2 + m.j) + N1 * k2) * sline);
+            }
             WDM_UNROLL
-            for (int k2 = 0; k2 < N2; ++k2)
-                X[i][k2] = apply_kf<KF, P, T>(X[i][k2], kl, kt, (i * N2 + m.j) + N1 * k2);
+            for (int k2 = 0; k2 < N2; ++k2) {
+                const int k = (i * N2 + m.j) + N1 * k2;
+                X[i][k2] = apply_kf<KF, P, T>(X[i][k2], kl, kt, k);
+                if constexpr (MODE == COL_FMIA) {
+                    const double w = kt.wc * (kco + k2l_s[k]);
+                    acc += w * (double(X[i][k2].x) * double(q[k2].y) - double(X[i][k2].y) * double(q[k2].x));
+                }
+            }
+        }
+        if constexpr (MODE == COL_FMIA) {
+            double wsum, wmax;
+            wave_sum_max<T>(&xbuf[m.wv][0], m.lane, ok ? acc : 0.0, 0.0, wsum, wmax);
+            if (m.lane == 0 && o < nother) kt.partials[wl] = wsum;
+        } else {
+            (void)acc;
+            (void)kco;
         }
         inv2<F, +1, P>(X, v, m.j, xb, tw);
         if (ok) {

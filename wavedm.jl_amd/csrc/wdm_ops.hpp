@@ -134,7 +134,8 @@ struct Launch {
         // 4 waves per workgroup: 4 staging buffers + transpose buffers fit
         // the 160 KB of LDS of one CU
         constexpr int SNW = 4;
-        if constexpr (!WDM_COL_STAGED || !S::OK || MODE == COL_FMIS || !std::is_same<T, float>::value) {
+        if constexpr (!WDM_COL_STAGED || !S::OK || MODE == COL_FMIS || MODE == COL_FMIA ||
+                      !std::is_same<T, float>::value) {
             return false;
         } else {
             // a lane copies E adjacent columns: the last one read must stay
@@ -154,7 +155,7 @@ struct Launch {
     static void col_t(cx<T>* d, int ncols, int nother, unsigned sl, unsigned so, const TwE* tw, KTabs kt,
                       hipStream_t s) {
         if (col_staged<PsiP, MODE, KF>(d, ncols, nother, sl, so, tw, kt, s)) return;
-        if constexpr (WDM_COL_TPW > 1 && MODE != COL_FMIS) {
+        if constexpr (WDM_COL_TPW > 1 && MODE != COL_FMIS && MODE != COL_FMIA) {
             WDM_LAUNCH((k_colp<T, PsiP, F, MODE, KF, NW, WDM_COL_TPW>),
                        dim3(col_blocks<NW * WDM_COL_TPW, F>(ncols, nother)), dim3(NW * 64), s, d, ncols, nother, sl,
                        so, tw, kt);
@@ -196,6 +197,7 @@ struct Launch {
             case COL_INV: col_t<COL_INV, KF_NONE>(d, ncols, nother, sl, so, tw, kt, s); break;
             case COL_FMI: col_t<COL_FMI, KF_DRIFT>(d, ncols, nother, sl, so, tw, kt, s); break;
             case COL_FMIS: col_t<COL_FMIS, KF_DRIFT>(d, ncols, nother, sl, so, tw, kt, s); break;
+            case COL_FMIA: col_t<COL_FMIA, KF_DRIFT>(d, ncols, nother, sl, so, tw, kt, s); break;
             case COL_MI: col_t<COL_MI, KF_DRIFT>(d, ncols, nother, sl, so, tw, kt, s); break;
         }
     }
