@@ -353,6 +353,19 @@ WDM_FI const E* stage_table(E* dst, const E* __restrict__ src) {
     return dst;
 }
 
+// The twiddles in the conflict-free layout (TL = 1 of fwd2/inv2): 2N
+// entries gathered from the plain table.
+template <class F, class E>
+WDM_FI const E* stage_tw2(E* dst, const E* __restrict__ src) {
+    for (int i = int(threadIdx.x); i < 2 * F::N; i += int(blockDim.x)) {
+        const int f = i < F::N ? i : i - F::N;
+        const int e = i < F::N ? (f % F::N2) * (f / F::N2) : (f % F::N1) * (f / F::N1);
+        dst[i] = src[e];
+    }
+    __syncthreads();
+    return dst;
+}
+
 struct LaneMap {
     int lane, wv, j, lw, w;
 };
@@ -664,8 +677,18 @@ WDM_FI void col_body(cx<T>* __restrict__ data, int ncols, int nother, unsigned s
                      const typename TwEntry<T>::type* __restrict__ tw, KTabs kt, unsigned bid) {
     constexpr int N1 = F::N1, N2 = F::N2, M = F::M;
     constexpr bool EARLY = WDM_COL_EARLY_LOADS && MODE != COL_MI;
+    // The FP32 fused drift passes of 48+ values per lane take their twiddles
+    // in the conflict-free layout: its offsets are compile-time constants
+    // per lane (k1 N2 + j), which frees the registers the products j k1 held
+    // in these register-capped passes (z drift at 384^3: 270 -> 257 us, the
+    // cap's spill 40 -> 8 bytes).  Elsewhere the gather that builds the
+    // layout costs more than it saves (y passes +9 us, Poisson z +5 us).
+    constexpr int TL = (std::is_same<T, float>::value && KF == KF_DRIFT && F::N1 >= 48 &&
+                        (MODE == COL_FMI || MODE == COL_FMIS || MODE == COL_FMIA))
+                           ? 1
+                           : 0;
     WDM_SHARED cx<T> xbuf[NW][F::XBUF];
-    WDM_SHARED typename TwEntry<T>::type tw_s[F::N];
+    WDM_SHARED typename TwEntry<T>::type tw_s[TL ? 2 * F::N : F::N];
     const LaneMap m = lane_map<F, NW>(bid);
     const int ntiles = (ncols + F::LPW - 1) / F::LPW;
     const int wl = int(bid) * NW + m.wv;
@@ -697,7 +720,8 @@ WDM_FI void col_body(cx<T>* __restrict__ data, int ncols, int nother, unsigned s
     if constexpr (MODE == COL_FMIA) {
         for (int i = int(threadIdx.x); i < F::N; i += int(blockDim.x)) k2l_s[i] = kt.k2[2][i];
     }
-    tw = stage_table<F::N>(tw_s, tw);
+    if constexpr (TL) tw = stage_tw2<F>(tw_s, tw);
+    else tw = stage_table<F::N>(tw_s, tw);
     const KfLane<KF, T> kl = kf_lane<KF, T>(kt, cc, oc);
     cx<T>* xb = &xbuf[m.wv][m.lw * F::XS];
     cx<T> X[M][N2];
@@ -713,7 +737,7 @@ WDM_FI void col_body(cx<T>* __restrict__ data, int ncols, int nother, unsigned s
     }
     constexpr bool FMI = MODE == COL_FMI || MODE == COL_FMIS || MODE == COL_FMIA;
     constexpr int DIR = (MODE == COL_FWD || FMI) ? -1 : +1;
-    fwd2<F, DIR, P>(v, X, m.j, xb, tw);
+    fwd2<F, DIR, P, TL>(v, X, m.j, xb, tw);
     if constexpr (FMI) {
         if (MODE == COL_FMIS && ok) {
             cx<T>* so = static_cast<cx<T>*>(kt.spec_out);
@@ -756,7 +780,7 @@ WDM_FI void col_body(cx<T>* __restrict__ data, int ncols, int nother, unsigned s
             (void)acc;
             (void)kco;
         }
-        inv2<F, +1, P>(X, v, m.j, xb, tw);
+        inv2<F, +1, P, TL>(X, v, m.j, xb, tw);
         if (ok) {
             const unsigned sbase = unsigned(oc) * sother + unsigned(cc) + unsigned(m.j) * sline;
             WDM_UNROLL

@@ -52,9 +52,25 @@ WDM_FI void xpose(cx<T>* __restrict__ xb, int me, const cx<T>* out, cx<T>* in) {
     }
 }
 
-// Forward-layout two-level FFT (see file comment).  `tw` is the length-N
-// twiddle table for this axis; `xb` the line's slice of the wave LDS buffer.
-template <class F, int DIR, class P, class T>
+// Twiddle table layouts (the FFTs read W^(j k1) in fwd2 and W^(q kp) in inv2):
+//   TL = 0: the plain length-N table, W^e at e;
+//   TL = 1: 2N entries, W^(j k1) at k1 N2 + j and W^(q kp) at N + kp N1 + q,
+//           so the N2 lanes of a line read consecutive entries (no LDS bank
+//           conflicts; the plain table puts lanes j = 0..N2-1 on the same
+//           banks whenever k1 is even).
+template <class F, int TL>
+WDM_FI int tw_fwd_index(int j, int k1) {
+    return TL ? k1 * F::N2 + j : j * k1;
+}
+template <class F, int TL>
+WDM_FI int tw_inv_index(int q, int kp) {
+    return TL ? F::N + kp * F::N1 + q : q * kp;
+}
+
+// Forward-layout two-level FFT (see file comment).  `tw` is the twiddle
+// table for this axis in layout TL; `xb` the line's slice of the wave LDS
+// buffer.
+template <class F, int DIR, class P, int TL = 0, class T>
 WDM_FI void fwd2(cx<T>* v, cx<T> (*X)[F::N2], int j, cx<T>* __restrict__ xb,
                  const typename TwEntry<T>::type* __restrict__ tw) {
     constexpr int N1 = F::N1, N2 = F::N2, M = F::M;
@@ -68,7 +84,7 @@ WDM_FI void fwd2(cx<T>* v, cx<T> (*X)[F::N2], int j, cx<T>* __restrict__ xb,
             for (int b = 0; b < N2; ++b) {
                 const int k1 = i * N2 + b;
                 if (k1 > 0)
-                    v[k1] = mul_tab<DIR, P>(v[k1], tw[j * k1]);
+                    v[k1] = mul_tab<DIR, P>(v[k1], tw[tw_fwd_index<F, TL>(j, k1)]);
             }
         }
         cx<T> in[N2];
@@ -81,7 +97,7 @@ WDM_FI void fwd2(cx<T>* v, cx<T> (*X)[F::N2], int j, cx<T>* __restrict__ xb,
 
 // Inverse-layout two-level FFT: lane j holds items q = i*N2 + j with
 // X[i][p] = x_hat[q + N1*p]; returns v[kq] = x[N2*kq + j].
-template <class F, int DIR, class P, class T>
+template <class F, int DIR, class P, int TL = 0, class T>
 WDM_FI void inv2(cx<T> (*X)[F::N2], cx<T>* v, int j, cx<T>* __restrict__ xb,
                  const typename TwEntry<T>::type* __restrict__ tw) {
     constexpr int N1 = F::N1, N2 = F::N2, M = F::M;
@@ -93,7 +109,7 @@ WDM_FI void inv2(cx<T> (*X)[F::N2], cx<T>* v, int j, cx<T>* __restrict__ xb,
             const int q = i * N2 + j;
             WDM_UNROLL
             for (int kp = 1; kp < N2; ++kp)
-                X[i][kp] = mul_tab<DIR, P>(X[i][kp], tw[q * kp]);
+                X[i][kp] = mul_tab<DIR, P>(X[i][kp], tw[tw_inv_index<F, TL>(q, kp)]);
         }
     }
     WDM_UNROLL
