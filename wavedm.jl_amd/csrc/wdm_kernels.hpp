@@ -240,7 +240,9 @@ WDM_FI double reduce_quadrant(double theta, int& q) {
     const double n = __builtin_rint(theta * kTwoOverPi);
     double r = fmaT(-n, kPio2Hi, theta);
     r = fmaT(-n, kPio2Lo, r);
-    q = int(long(n) & 3);
+    // n mod 4 in FP64 (exact for any integer-valued n below 2^53), then a
+    // 32-bit conversion of 0..3 (no 64-bit integer conversion)
+    q = int(fmaT(-4.0, __builtin_floor(n * 0.25), n));
     return r;
 }
 
@@ -318,9 +320,11 @@ WDM_FI cx<float> kick_rot(cx<float> a, double theta) {
     const float r = float(reduce_quadrant(theta, q));
     float s, c;
     sincos_small(r, s, c);
-    // rotation by phi = -r
+    // rotation by phi = -r; t = tan(phi/2) = -s / (1 + c) with the hardware
+    // reciprocal (1 + c >= 1.7 here): its ulp-level error only perturbs the
+    // angle, the shears keep the determinant exactly 1
     const float sp = -s;
-    const float tp = -s / (1.0f + c);
+    const float tp = -s * rcpf(1.0f + c);
     float x = a.x, y = a.y;
     x = fmaT(-tp, y, x);
     y = fmaT(sp, x, y);
