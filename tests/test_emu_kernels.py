@@ -15,7 +15,7 @@ SUBSET = ("roundtrip or (poisson and (8x8x8 or 32x16x8 or 160x16x8 or 12x8x384))
           "(step_fp64 and (16x16x16 or 128x8x16 or 8x384x16)) or kick_and_drift or strang or "
           "quadrupole or static_potential or nls or reductions or error_paths or "
           "(adjoint_matches_oracle or checkpoint_interval or adjoint_error_paths or adjoint_tape or (quadrupole_gradients and 16x12x8)) or "
-          "ensemble_equals or recaptures or time_dependent or (every_variant and (64x8x8 or 32x16x8)) or "
+          "ensemble_equals or recaptures or time_dependent or (every_variant and (64x8x8 or 32x16x8 or 8x8x384 or 8x8x192)) or "
           "device_matches_golden or (diagnostics_match_oracle and (32x24x16 or 16x12x8 or 96x48x12)) or "
           "diagnostics_after or diagnostics_error or restart_continues or restart_with_mass or mass_reference or "
           "density_average or "
