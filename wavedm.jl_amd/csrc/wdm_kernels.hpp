@@ -320,11 +320,11 @@ WDM_FI cx<float> kick_rot(cx<float> a, double theta) {
     const float r = float(reduce_quadrant(theta, q));
     float s, c;
     sincos_small(r, s, c);
-    // rotation by phi = -r; t = tan(phi/2) = -s / (1 + c) with the hardware
-    // reciprocal (1 + c >= 1.7 here): its ulp-level error only perturbs the
-    // angle, the shears keep the determinant exactly 1
+    // rotation by phi = -r; t = tan(phi/2) = -s / (1 + c), correctly rounded
+    // (with the hardware reciprocal instead, the 384^3 FP32 run's mass drift
+    // over 1000 steps grew from 7.6e-8 to 1.0e-7 for a 0.5% faster kick)
     const float sp = -s;
-    const float tp = -s * rcpf(1.0f + c);
+    const float tp = -s / (1.0f + c);
     float x = a.x, y = a.y;
     x = fmaT(-tp, y, x);
     y = fmaT(sp, x, y);
