@@ -215,20 +215,7 @@ void k_adj_xg(cx<T>* __restrict__ g, const cx<T>* __restrict__ psi_n, const cx<T
     T u[N1];
     if (uhat) {
         const unsigned hbase = unsigned(lc) * unsigned(hpitch);
-        WDM_UNROLL
-        for (int i = 0; i < M; ++i)
-            WDM_UNROLL
-            for (int p = 0; p < N2; ++p) {
-                const int k = i * N2 + m.j + N1 * p;
-                cx<T> zz;
-                if (2 * k <= N) {
-                    zz = WDM_LDG(ok, uhat, hbase + unsigned(k));
-                } else {
-                    const cx<T> hh = WDM_LDG(ok, uhat, hbase + unsigned(N - k));
-                    zz = mk(hh.x, -hh.y);
-                }
-                X[i][p] = zz;
-            }
+        hermitian_row<F>(uhat, hbase, ok, m, &xbuf[m.wv][0], X);
         inv2<F, +1, PoissonPolicy>(X, v, m.j, xb, tw);
         WDM_UNROLL
         for (int kq = 0; kq < N1; ++kq) u[kq] = v[kq].x;

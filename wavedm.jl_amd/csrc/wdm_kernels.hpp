@@ -1185,25 +1185,10 @@ void k_x_c2r_real(const cx<T>* __restrict__ hsrc, double* __restrict__ dst, int 
     const int l = m.w * F::LPW + m.lw;
     const bool ok = l < nlines;
     const int lc = ok ? l : 0;
-    const cx<T>* h = hsrc + size_t(lc) * hpitch;
     cx<T>* xb = &xbuf[m.wv][m.lw * F::XS];
-    const cx<T> zero = mk(T(0), T(0));
     cx<T> X[M][N2];
     cx<T> v[N1];
-    WDM_UNROLL
-    for (int i = 0; i < M; ++i)
-        WDM_UNROLL
-        for (int p = 0; p < N2; ++p) {
-            const int k = i * N2 + m.j + N1 * p;
-            cx<T> zz;
-            if (2 * k <= N) {
-                zz = h[k];
-            } else {
-                const cx<T> hh = h[N - k];
-                zz = mk(hh.x, -hh.y);
-            }
-            X[i][p] = ok ? zz : zero;
-        }
+    hermitian_row<F>(hsrc, unsigned(lc) * unsigned(hpitch), ok, m, &xbuf[m.wv][0], X);
     inv2<F, +1, PoissonPolicy>(X, v, m.j, xb, tw);
     if (ok) {
         double* row = dst + size_t(lc) * N;
@@ -1299,20 +1284,7 @@ void k_x_red(const cx<T>* __restrict__ psi, const cx<T>* __restrict__ phihat, in
     T phi[N1];
     if (phihat) {
         const unsigned hbase = unsigned(lc) * unsigned(hpitch);
-        WDM_UNROLL
-        for (int i = 0; i < M; ++i)
-            WDM_UNROLL
-            for (int p = 0; p < N2; ++p) {
-                const int k = i * N2 + m.j + N1 * p;
-                cx<T> zz;
-                if (2 * k <= N) {
-                    zz = WDM_LDG(ok, phihat, hbase + unsigned(k));
-                } else {
-                    const cx<T> hh = WDM_LDG(ok, phihat, hbase + unsigned(N - k));
-                    zz = mk(hh.x, -hh.y);
-                }
-                X[i][p] = zz;
-            }
+        hermitian_row<F>(phihat, hbase, ok, m, &xbuf[m.wv][0], X);
         inv2<F, +1, PoissonPolicy>(X, v, m.j, xb, tw);
         WDM_UNROLL
         for (int kq = 0; kq < N1; ++kq) phi[kq] = v[kq].x;
