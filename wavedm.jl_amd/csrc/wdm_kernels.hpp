@@ -1066,7 +1066,13 @@ WDM_FI void density_r2c(const cx<T>* v, bool ok, const LaneMap& m, cx<T>* xb, cx
 // Inverse x-FFT (+ absorber along x) of psi rows, store psi in real space,
 // then rho = |psi|^2 -> R2C_x -> rho_hat.  `border` is a length-Nx FP64
 // vector or null (src/solver/KDK.jl:95-96; broadcast along x, Q4).
-template <class T, class P, class F, int NW>
+// NTS: psi is read and written with the streaming (non-temporal) hint, so
+// that it does not displace the half spectrum this pass writes from the
+// memory-side cache (MI355X: 256 MB Infinity Cache) before the Poisson
+// passes reread it.  Worth it only when psi is larger than that cache and the
+// half spectrum is not (384^3 FP32: the step 1.540 -> 1.515 ms; at 256^3,
+// where psi fits, it costs 7%).
+template <class T, class P, class F, int NW, bool NTS = false>
 WDM_FI void x_inv_dens_body(cx<T>* __restrict__ psi, cx<T>* __restrict__ rhohat, int nlines, int hpitch,
                             const typename TwEntry<T>::type* __restrict__ tw, const double* __restrict__ border,
                             double* __restrict__ partials, unsigned bid) {
@@ -1086,7 +1092,11 @@ WDM_FI void x_inv_dens_body(cx<T>* __restrict__ psi, cx<T>* __restrict__ rhohat,
     WDM_UNROLL
     for (int i = 0; i < M; ++i)
         WDM_UNROLL
-        for (int p = 0; p < N2; ++p) X[i][p] = WDM_LDG(ok, psi, rbase + unsigned((i * N2 + m.j) + N1 * p));
+        for (int p = 0; p < N2; ++p) {
+            const unsigned e = rbase + unsigned((i * N2 + m.j) + N1 * p);
+            if constexpr (NTS) X[i][p] = ok ? ldg_nt(psi, e) : zero;
+            else X[i][p] = WDM_LDG(ok, psi, e);
+        }
     inv2<F, +1, P>(X, v, m.j, xb, tw);
     if (border) {
         WDM_UNROLL
@@ -1094,18 +1104,21 @@ WDM_FI void x_inv_dens_body(cx<T>* __restrict__ psi, cx<T>* __restrict__ rhohat,
     }
     if (ok) {
         WDM_UNROLL
-        for (int kq = 0; kq < N1; ++kq) stg(psi, rbase + unsigned(N2 * kq + m.j), v[kq]);
+        for (int kq = 0; kq < N1; ++kq) {
+            if constexpr (NTS) stg_nt(psi, rbase + unsigned(N2 * kq + m.j), v[kq]);
+            else stg(psi, rbase + unsigned(N2 * kq + m.j), v[kq]);
+        }
     }
     if (rhohat) {
         density_r2c<T, F, NW>(v, ok, m, xb, &xbuf[m.wv][0], tw, rhohat, unsigned(lc) * unsigned(hpitch), partials);
     }
 }
-template <class T, class P, class F, int NW>
+template <class T, class P, class F, int NW, bool NTS = false>
 WDM_KERNEL_W(NW * 64, (std::is_same<T, float>::value ? WDM_MINW_XINV_F32 : 1))
 void k_x_inv_dens(cx<T>* __restrict__ psi, cx<T>* __restrict__ rhohat, int nlines, int hpitch,
                   const typename TwEntry<T>::type* __restrict__ tw, const double* __restrict__ border,
                   double* __restrict__ partials) {
-    x_inv_dens_body<T, P, F, NW>(psi, rhohat, nlines, hpitch, tw, border, partials, blockIdx.x);
+    x_inv_dens_body<T, P, F, NW, NTS>(psi, rhohat, nlines, hpitch, tw, border, partials, blockIdx.x);
 }
 template <class T, class P, class F, int NW>
 WDM_KERNEL_W(NW * 64, (std::is_same<T, float>::value ? WDM_MINW_XINV_F32 : 1))

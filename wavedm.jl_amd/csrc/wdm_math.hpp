@@ -43,6 +43,33 @@ WDM_HDFI void stg(T* base, unsigned idx, const T& v) {
     *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + idx * unsigned(sizeof(T))) = v;
 }
 
+// The same with the non-temporal (streaming) hint: for data that the next
+// pass does not reread from the caches, so that it does not displace data the
+// next pass does reread.
+template <class T>
+WDM_HDFI cx<T> ldg_nt(const cx<T>* base, unsigned idx) {
+#ifdef WDM_GPU
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    const v2 r = __builtin_nontemporal_load(
+        reinterpret_cast<const v2*>(reinterpret_cast<const char*>(base) + idx * unsigned(sizeof(cx<T>))));
+    return mk(r.x, r.y);
+#else
+    return ldg(base, idx);
+#endif
+}
+template <class T>
+WDM_HDFI void stg_nt(cx<T>* base, unsigned idx, const cx<T>& v) {
+#ifdef WDM_GPU
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    v2 r;
+    r.x = v.x;
+    r.y = v.y;
+    __builtin_nontemporal_store(r, reinterpret_cast<v2*>(reinterpret_cast<char*>(base) + idx * unsigned(sizeof(cx<T>))));
+#else
+    stg(base, idx, v);
+#endif
+}
+
 WDM_HDFI float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 WDM_HDFI double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
 

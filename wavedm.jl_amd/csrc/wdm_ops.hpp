@@ -83,6 +83,9 @@ struct AxisOps {
                    const KickArgs& ka, hipStream_t s);  // kernel picked by the potential kind of ka.vm
     void (*x_inv_dens)(cx<T>* psi, cx<T>* rhohat, int nlines, int hpitch, const TwE* tw, const double* border,
                        double* partials, hipStream_t s);
+    // the same with psi streamed past the caches (k_x_inv_dens NTS)
+    void (*x_inv_dens_stream)(cx<T>* psi, cx<T>* rhohat, int nlines, int hpitch, const TwE* tw,
+                              const double* border, double* partials, hipStream_t s);
     void (*x_dens)(const cx<T>* psi, cx<T>* rhohat, int nlines, int hpitch, const TwE* tw, double* partials,
                    hipStream_t s);
     void (*x_r2c_real)(const double* src, cx<T>* rhohat, int nlines, int hpitch, const TwE* tw, hipStream_t s);
@@ -225,6 +228,11 @@ struct Launch {
         WDM_LAUNCH((k_x_inv_dens<T, PsiP, F, NW>), dim3(x_blocks<NW, F>(nlines)), dim3(NW * 64), s, psi, rh, nlines,
                    hp, tw, border, partials);
     }
+    static void x_inv_dens_stream(cx<T>* psi, cx<T>* rh, int nlines, int hp, const TwE* tw, const double* border,
+                                  double* partials, hipStream_t s) {
+        WDM_LAUNCH((k_x_inv_dens<T, PsiP, F, NW, true>), dim3(x_blocks<NW, F>(nlines)), dim3(NW * 64), s, psi, rh,
+                   nlines, hp, tw, border, partials);
+    }
     static void x_dens(const cx<T>* psi, cx<T>* rh, int nlines, int hp, const TwE* tw, double* partials,
                        hipStream_t s) {
         WDM_LAUNCH((k_x_dens<T, F, NW>), dim3(x_blocks<NW, F>(nlines)), dim3(NW * 64), s, psi, rh, nlines, hp, tw,
@@ -317,8 +325,8 @@ struct Launch {
     }
     static const AxisOps<T>* table() {
         static const AxisOps<T> ops = {F::N, F::N1, F::N2, F::LPW, NW, &col, &x_kick, &x_inv_dens,
-                                       &x_dens, &x_r2c_real, &x_c2r_real, &x_fft, &adj_xs, &adj_xg,
-                                       &x_red, &col_red, &col_b, &x_kick_b, &x_inv_dens_b};
+                                       &x_inv_dens_stream, &x_dens, &x_r2c_real, &x_c2r_real, &x_fft,
+                                       &adj_xs, &adj_xg, &x_red, &col_red, &col_b, &x_kick_b, &x_inv_dens_b};
         return &ops;
     }
 };

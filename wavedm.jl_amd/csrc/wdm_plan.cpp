@@ -129,6 +129,10 @@ struct wdm_plan {
     double four_pi_G = 4.0 * M_PI;
     bool self_gravity = true;
     int hpitch = 0;
+    // the inverse x pass streams psi past the caches (psi larger than the
+    // memory-side cache, the half spectrum not: it then stays cached for the
+    // Poisson passes)
+    bool stream_psi = false;
     hipStream_t stream = nullptr;
     AxisState ax[3];
     DBuf psi, rhohat, partials, redout, border, vstatic, scratch, scratch_real;
@@ -513,9 +517,9 @@ void x_inv_close(wdm_plan* p) {
     typedef typename TwEntry<T>::type TwE;
     const AxisOps<T>* ox = ops_of<T>(p, 0);
     arm(p, WDM_PASS_XINV);
-    ox->x_inv_dens(p->psi.as<cx<T>>(), p->self_gravity ? p->rhohat.as<cx<T>>() : nullptr, p->ny * p->nz,
-                   p->hpitch, p->ax[0].tw.as<TwE>(), p->border_on ? p->border.as<double>() : nullptr,
-                   p->partials.as<double>(), p->stream);
+    (p->stream_psi && p->self_gravity ? ox->x_inv_dens_stream : ox->x_inv_dens)(
+        p->psi.as<cx<T>>(), p->self_gravity ? p->rhohat.as<cx<T>>() : nullptr, p->ny * p->nz, p->hpitch,
+        p->ax[0].tw.as<TwE>(), p->border_on ? p->border.as<double>() : nullptr, p->partials.as<double>(), p->stream);
     hip_check(hipGetLastError(), "inverse x pass");
     mark(p, WDM_PASS_XINV);
     p->kspace = false;
@@ -1114,6 +1118,13 @@ void init_plan(wdm_plan* p) {
     // slower or no faster for the Poisson column passes at 384^3)
     p->hpitch = ((p->nx / 2 + 1) + WDM_HPITCH_ALIGN - 1) / WDM_HPITCH_ALIGN * WDM_HPITCH_ALIGN;
     p->rhohat.get<cx<T>>(size_t(p->hpitch) * p->ny * p->nz);
+    {
+        // MI355X's Infinity Cache (memory-side, shared by all XCDs)
+        constexpr size_t kMallBytes = size_t(256) << 20;
+        const size_t psi_bytes = sizeof(cx<T>) * p->ncell;
+        const size_t hat_bytes = sizeof(cx<T>) * size_t(p->hpitch) * p->ny * p->nz;
+        p->stream_psi = psi_bytes > kMallBytes && hat_bytes < kMallBytes;
+    }
     update_partials<T>(p);
     p->redout.get<double>(RS_COUNT + 2);
 }
