@@ -660,9 +660,13 @@ void k_colp(cx<T>* __restrict__ data, int ncols, int nother, unsigned sline, uns
 #ifndef WDM_MINW_COL_F32
 #define WDM_MINW_COL_F32 1
 #endif
-template <class T, int MODE, int KF>
+template <class T, class F, int MODE, int KF>
 constexpr int col_min_waves() {
     if (!std::is_same<T, float>::value) return 1;
+    // the Poisson z pass at 48+ values per lane, with the conflict-free
+    // twiddle layout, fits three waves (384^3: 138 -> 131 us); shorter lines
+    // keep the uncapped build
+    if (MODE == COL_FMI && KF == KF_POISSON && F::N1 >= 48) return 3;
     if (MODE == COL_FMI) return KF == KF_POISSON ? WDM_MINW_PFMI_F32 : WDM_MINW_FMI_F32;
     if (MODE == COL_FWD || MODE == COL_INV) return WDM_MINW_COL_F32;
     return 1;
@@ -681,13 +685,14 @@ WDM_FI void col_body(cx<T>* __restrict__ data, int ncols, int nother, unsigned s
                      const typename TwEntry<T>::type* __restrict__ tw, KTabs kt, unsigned bid) {
     constexpr int N1 = F::N1, N2 = F::N2, M = F::M;
     constexpr bool EARLY = WDM_COL_EARLY_LOADS && MODE != COL_MI;
-    // The FP32 fused drift passes of 48+ values per lane take their twiddles
-    // in the conflict-free layout: its offsets are compile-time constants
-    // per lane (k1 N2 + j), which frees the registers the products j k1 held
-    // in these register-capped passes (z drift at 384^3: 270 -> 257 us, the
-    // cap's spill 40 -> 8 bytes).  Elsewhere the gather that builds the
-    // layout costs more than it saves (y passes +9 us, Poisson z +5 us).
-    constexpr int TL = (std::is_same<T, float>::value && KF == KF_DRIFT && F::N1 >= 48 &&
+    // The FP32 fused passes (drift and Poisson) of 48+ values per lane take
+    // their twiddles in the conflict-free layout: its offsets are
+    // compile-time constants per lane (k1 N2 + j), which frees the registers
+    // the products j k1 held in these register-capped passes (z drift at
+    // 384^3: 270 -> 257 us, the cap's spill 40 -> 8 bytes; the Poisson z
+    // pass fits three waves with it).  Elsewhere the gather that builds the
+    // layout costs more than it saves (y passes +9 us).
+    constexpr int TL = (std::is_same<T, float>::value && (KF == KF_DRIFT || KF == KF_POISSON) && F::N1 >= 48 &&
                         (MODE == COL_FMI || MODE == COL_FMIS || MODE == COL_FMIA))
                            ? 1
                            : 0;
@@ -803,7 +808,7 @@ WDM_FI void col_body(cx<T>* __restrict__ data, int ncols, int nother, unsigned s
 }
 
 template <class T, class P, class F, int MODE, int KF, int NW>
-WDM_KERNEL_W(NW * 64, (col_min_waves<T, MODE, KF>()))
+WDM_KERNEL_W(NW * 64, (col_min_waves<T, F, MODE, KF>()))
 void k_col(cx<T>* __restrict__ data, int ncols, int nother, unsigned sline, unsigned sother,
            const typename TwEntry<T>::type* __restrict__ tw, KTabs kt) {
     col_body<T, P, F, MODE, KF, NW>(data, ncols, nother, sline, sother, tw, kt, xcd_block());
@@ -816,7 +821,7 @@ struct ColMember {
     KTabs kt;
 };
 template <class T, class P, class F, int MODE, int KF, int NW>
-WDM_KERNEL_W(NW * 64, (col_min_waves<T, MODE, KF>()))
+WDM_KERNEL_W(NW * 64, (col_min_waves<T, F, MODE, KF>()))
 void k_col_b(const ColMember<T>* __restrict__ mem, int ncols, int nother, unsigned sline, unsigned sother,
              const typename TwEntry<T>::type* __restrict__ tw) {
     unsigned member, b;

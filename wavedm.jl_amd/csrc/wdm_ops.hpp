@@ -163,7 +163,7 @@ struct Launch {
                        dim3(col_blocks<NW * WDM_COL_TPW, F>(ncols, nother)), dim3(NW * 64), s, d, ncols, nother, sl,
                        so, tw, kt);
         } else {
-            constexpr int W = (MODE == COL_FMI && col_min_waves<T, MODE, KF>() > 1) ? WDM_NW_FMI : NW;
+            constexpr int W = (MODE == COL_FMI && col_min_waves<T, F, MODE, KF>() > 1) ? WDM_NW_FMI : NW;
             WDM_LAUNCH((k_col<T, PsiP, F, MODE, KF, W>), dim3(col_blocks<W, F>(ncols, nother)), dim3(W * 64), s,
                        d, ncols, nother, sl, so, tw, kt);
         }
@@ -178,7 +178,7 @@ struct Launch {
                        dim3(col_blocks<NW * WDM_COL_TPW, F>(ncols, nother)), dim3(NW * 64), s, d, ncols, nother, sl,
                        so, tw, kt);
         } else {
-            constexpr int W = (MODE == COL_FMI && col_min_waves<T, MODE, KF>() > 1) ? WDM_NW_FMI : NW;
+            constexpr int W = (MODE == COL_FMI && col_min_waves<T, F, MODE, KF>() > 1) ? WDM_NW_FMI : NW;
             WDM_LAUNCH((k_col<T, PoisP, F, MODE, KF, W>), dim3(col_blocks<W, F>(ncols, nother)), dim3(W * 64), s,
                        d, ncols, nother, sl, so, tw, kt);
         }
@@ -287,7 +287,7 @@ struct Launch {
     template <class PP, int MODE, int KF>
     static void colb_t(const ColMember<T>* mem, int nmem, int ncols, int nother, unsigned sl, unsigned so,
                        const TwE* tw, hipStream_t s) {
-        constexpr int W = (MODE == COL_FMI && col_min_waves<T, MODE, KF>() > 1) ? WDM_NW_FMI : NW;
+        constexpr int W = (MODE == COL_FMI && col_min_waves<T, F, MODE, KF>() > 1) ? WDM_NW_FMI : NW;
         WDM_LAUNCH((k_col_b<T, PP, F, MODE, KF, W>), dim3(col_blocks<W, F>(ncols, nother), unsigned(nmem)),
                    dim3(W * 64), s, mem, ncols, nother, sl, so, tw);
     }
