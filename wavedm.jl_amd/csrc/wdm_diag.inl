@@ -193,7 +193,12 @@ void k_diag_hist(const cx<T>* __restrict__ psi, int nx, int ny, int nz, const do
         const double dy = ys[y] - c1, dz = zs[z] - c2;
         const double r2yz = dy * dy + dz * dz;
         const unsigned base = line * unsigned(nx);
-        for (int x = lane; x < nx; x += kWave) {
+        // lanes take cells nx/64 apart (when nx is a multiple of 64): cells
+        // next to each other along x often fall in one bin, and 64 lanes
+        // adding to one LDS counter serialise
+        const int chunk = nx % kWave == 0 ? nx / kWave : 0;
+        for (int xi = lane; xi < nx; xi += kWave) {
+            const int x = chunk ? (xi % kWave) * chunk + xi / kWave : xi;
             const double rho = rho_of(psi[base + unsigned(x)]);
             if (rho > 0.0) {  // zero weights take no part (StatsBase drops them)
                 const double dx = xs[x] - c0;
