@@ -47,10 +47,18 @@ constexpr int kNW = sizeof(T) == 4 ? WDM_NW_F32 : WDM_NW_F64;
 #define WDM_COL_TPW 1
 #endif
 // workgroup size of the fused z passes when they ask for more than one wave
-// per SIMD (a multiple of the waves per SIMD wastes no slots)
+// per SIMD: 4 for the drift pass, 2 for the Poisson pass (384^3 FP32: drift
+// 256 us at 4, 291 at 2 or 8; Poisson 125 us at 2, 131 at 4, 140 at 8)
 #ifndef WDM_NW_FMI
 #define WDM_NW_FMI 4
 #endif
+#ifndef WDM_NW_PFMI
+#define WDM_NW_PFMI 2
+#endif
+template <int MODE, int KF, int MINW, int NW>
+constexpr int fmi_group() {
+    return (MODE == COL_FMI && MINW > 1) ? (KF == KF_POISSON ? WDM_NW_PFMI : WDM_NW_FMI) : NW;
+}
 // LDS-staged persistent column passes (k_cols) where the tile shape allows
 #ifndef WDM_COL_STAGED
 #define WDM_COL_STAGED 0
@@ -163,7 +171,7 @@ struct Launch {
                        dim3(col_blocks<NW * WDM_COL_TPW, F>(ncols, nother)), dim3(NW * 64), s, d, ncols, nother, sl,
                        so, tw, kt);
         } else {
-            constexpr int W = (MODE == COL_FMI && col_min_waves<T, F, MODE, KF>() > 1) ? WDM_NW_FMI : NW;
+            constexpr int W = fmi_group<MODE, KF, col_min_waves<T, F, MODE, KF>(), NW>();
             WDM_LAUNCH((k_col<T, PsiP, F, MODE, KF, W>), dim3(col_blocks<W, F>(ncols, nother)), dim3(W * 64), s,
                        d, ncols, nother, sl, so, tw, kt);
         }
@@ -178,7 +186,7 @@ struct Launch {
                        dim3(col_blocks<NW * WDM_COL_TPW, F>(ncols, nother)), dim3(NW * 64), s, d, ncols, nother, sl,
                        so, tw, kt);
         } else {
-            constexpr int W = (MODE == COL_FMI && col_min_waves<T, F, MODE, KF>() > 1) ? WDM_NW_FMI : NW;
+            constexpr int W = fmi_group<MODE, KF, col_min_waves<T, F, MODE, KF>(), NW>();
             WDM_LAUNCH((k_col<T, PoisP, F, MODE, KF, W>), dim3(col_blocks<W, F>(ncols, nother)), dim3(W * 64), s,
                        d, ncols, nother, sl, so, tw, kt);
         }
@@ -287,7 +295,7 @@ struct Launch {
     template <class PP, int MODE, int KF>
     static void colb_t(const ColMember<T>* mem, int nmem, int ncols, int nother, unsigned sl, unsigned so,
                        const TwE* tw, hipStream_t s) {
-        constexpr int W = (MODE == COL_FMI && col_min_waves<T, F, MODE, KF>() > 1) ? WDM_NW_FMI : NW;
+        constexpr int W = fmi_group<MODE, KF, col_min_waves<T, F, MODE, KF>(), NW>();
         WDM_LAUNCH((k_col_b<T, PP, F, MODE, KF, W>), dim3(col_blocks<W, F>(ncols, nother), unsigned(nmem)),
                    dim3(W * 64), s, mem, ncols, nother, sl, so, tw);
     }
