@@ -48,7 +48,7 @@ constexpr int kNW = sizeof(T) == 4 ? WDM_NW_F32 : WDM_NW_F64;
 #endif
 // workgroup size of the fused z passes when they ask for more than one wave
 // per SIMD: 4 for the drift pass, 2 for the Poisson pass (384^3 FP32: drift
-// 256 us at 4, 291 at 2 or 8; Poisson 125 us at 2, 131 at 4, 140 at 8)
+// 256 us at 4, 291 at 2 or 8; Poisson 125 us at 2, 131 at 4, 140 at 8, 143 at 1)
 #ifndef WDM_NW_FMI
 #define WDM_NW_FMI 4
 #endif
