@@ -1094,14 +1094,23 @@ WDM_FI void x_inv_dens_body(cx<T>* __restrict__ psi, cx<T>* __restrict__ rhohat,
     const cx<T> zero = mk(T(0), T(0));
     cx<T> X[M][N2];
     cx<T> v[N1];
-    WDM_UNROLL
-    for (int i = 0; i < M; ++i)
+    if constexpr (NTS) {
+        // Streaming loads in row order: the M loads of one p cover M * N2
+        // contiguous values, so the two 64-byte halves of each 128-byte line
+        // (FP32) are requested by consecutive loads.  Issued i-major, the
+        // second half comes N2 loads after the first, and with the streaming
+        // hint the pass fetched 1.58x the bytes of psi it reads (384^3 FP32:
+        // 247 -> 232 us in row order).
         WDM_UNROLL
-        for (int p = 0; p < N2; ++p) {
-            const unsigned e = rbase + unsigned((i * N2 + m.j) + N1 * p);
-            if constexpr (NTS) X[i][p] = ok ? ldg_nt(psi, e) : zero;
-            else X[i][p] = WDM_LDG(ok, psi, e);
-        }
+        for (int p = 0; p < N2; ++p)
+            WDM_UNROLL
+            for (int i = 0; i < M; ++i) X[i][p] = ok ? ldg_nt(psi, rbase + unsigned((i * N2 + m.j) + N1 * p)) : zero;
+    } else {
+        WDM_UNROLL
+        for (int i = 0; i < M; ++i)
+            WDM_UNROLL
+            for (int p = 0; p < N2; ++p) X[i][p] = WDM_LDG(ok, psi, rbase + unsigned((i * N2 + m.j) + N1 * p));
+    }
     inv2<F, +1, P>(X, v, m.j, xb, tw);
     if (border) {
         WDM_UNROLL
